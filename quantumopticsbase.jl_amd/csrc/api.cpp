@@ -1,0 +1,1558 @@
+// Host side of libqob_hip.so: contexts, operator handles, planning and the
+// mul! dispatch table of QuantumOpticsBase.jl, re-expressed as launches of
+// the kernels in k_*.hip.
+//
+// Reference dispatch (Julia multiple dispatch on 5-argument mul!):
+//   Operator x Operator/Ket/Bra      src/operators_dense.jl:394-396
+//   SparseOperator x ...             src/operators_sparse.jl:199-202
+//   LazyTensor x Ket/Bra/Operator    src/operators_lazytensor.jl:539-609
+//   LazySum x Ket/Bra/Operator       src/operators_lazysum.jl:189-238
+// Planning that the reference does per call (identity filtering, isometries,
+// strides, temporaries) is done once per operator and cached on its handle.
+#include "qob_internal.h"
+
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using cd = std::complex<double>;
+
+namespace qob {
+
+extern int g_sell_cj, g_csc_rb, g_csc_cc, g_csc_remap;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static qob_status fail(qob_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define QOB_HIP(x)                                                                              \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) return fail(QOB_EHIP, "%s: %s", #x, hipGetErrorString(e_));       \
+    } while (0)
+#define QOB_TRY(x)                          \
+    do {                                    \
+        qob_status st_ = (x);               \
+        if (st_ != QOB_OK) return st_;      \
+    } while (0)
+
+static inline double2 d2(cd z) { return make_double2(z.real(), z.imag()); }
+static inline double2 d2(qob_c64 z) { return make_double2(z.re, z.im); }
+static inline cd cdv(qob_c64 z) { return cd(z.re, z.im); }
+static inline bool is_zero(cd z) { return z.real() == 0.0 && z.imag() == 0.0; }
+static inline size_t esize(qob_dtype dt) { return dt == QOB_C64 ? 16 : 8; }
+
+// -------------------------------------------------------------- device buf
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    qob_status alloc(size_t b) {
+        if (b <= bytes && p) return QOB_OK;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        if (b == 0) b = 16;
+        if (hipMalloc(&p, b) != hipSuccess) {
+            p = nullptr;
+            return fail(QOB_ENOMEM, "hipMalloc of %zu bytes failed", b);
+        }
+        bytes = b;
+        return QOB_OK;
+    }
+    qob_status upload(const void *src, size_t b) {
+        QOB_TRY(alloc(b));
+        if (b) QOB_HIP(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
+        return QOB_OK;
+    }
+};
+
+// Upload complex values in the requested dtype.
+static qob_status upload_vals(DevBuf &buf, const std::vector<cd> &v, qob_dtype dt) {
+    if (dt == QOB_C64) {
+        std::vector<double2> h(v.size());
+        for (size_t i = 0; i < v.size(); ++i) h[i] = d2(v[i]);
+        return buf.upload(h.data(), h.size() * sizeof(double2));
+    }
+    std::vector<float2> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = make_float2((float)v[i].real(), (float)v[i].imag());
+    return buf.upload(h.data(), h.size() * sizeof(float2));
+}
+
+// ------------------------------------------------------------- row lists
+// Host sparse matrix as lists of rows: row i holds (idx[p], val[p]) for
+// p in [ptr[i], ptr[i+1]).
+struct RowLists {
+    int64_t nr = 0, nc = 0;
+    std::vector<int32_t> ptr, idx;
+    std::vector<cd> val;
+    bool monomial() const {
+        for (int64_t i = 0; i < nr; ++i)
+            if (ptr[i + 1] - ptr[i] > 1) return false;
+        return true;
+    }
+};
+
+// Transpose of row lists (rows of X -> rows of X^T), optionally conjugated.
+static RowLists transpose_rows(const RowLists &X, bool conj) {
+    RowLists T;
+    T.nr = X.nc;
+    T.nc = X.nr;
+    T.ptr.assign(T.nr + 1, 0);
+    for (int32_t c : X.idx) T.ptr[c + 1]++;
+    for (int64_t i = 0; i < T.nr; ++i) T.ptr[i + 1] += T.ptr[i];
+    T.idx.resize(X.idx.size());
+    T.val.resize(X.val.size());
+    std::vector<int32_t> pos(T.ptr.begin(), T.ptr.end() - 1);
+    for (int64_t r = 0; r < X.nr; ++r) {
+        for (int32_t p = X.ptr[r]; p < X.ptr[r + 1]; ++p) {
+            const int32_t c = X.idx[p];
+            T.idx[pos[c]] = (int32_t)r;
+            T.val[pos[c]] = conj ? std::conj(X.val[p]) : X.val[p];
+            pos[c]++;
+        }
+    }
+    return T;
+}
+
+static RowLists identity_rows(int64_t m, int64_t n) {
+    RowLists R;
+    R.nr = m;
+    R.nc = n;
+    R.ptr.assign(m + 1, 0);
+    const int64_t d = std::min(m, n);
+    for (int64_t i = 0; i < m; ++i) {
+        R.ptr[i + 1] = R.ptr[i] + (i < d ? 1 : 0);
+        if (i < d) {
+            R.idx.push_back((int32_t)i);
+            R.val.push_back(cd(1, 0));
+        }
+    }
+    return R;
+}
+
+// Variant of an operator's matrix: bit0 = transpose, bit1 = conjugate.
+// N = 0, T = 1, conj = 2, adjoint H = 3; composition is XOR.
+enum { V_N = 0, V_T = 1, V_CJ = 2, V_H = 3 };
+
+// --------------------------------------------------------------- sparse op
+struct SellTab {
+    DevBuf off, col, val;
+    SellDev dev;
+};
+struct CscTab {
+    DevBuf ptr, row, val;
+    CscDev dev;
+};
+
+struct SparseData {
+    int64_t m = 0, n = 0;
+    RowLists csc;  // columns of M as lists (rows of M^T)
+    RowLists csr;  // rows of M
+    std::mutex mu;
+    std::map<int, std::shared_ptr<SellTab>> sell;   // key: var*2 + dt
+    std::map<int, std::shared_ptr<CscTab>> cols;    // key: var*2 + dt
+
+    // rows of op_v(M)
+    RowLists rows(int v) const {
+        const bool t = v & 1, c = (v >> 1) & 1;
+        const RowLists &src = t ? csc : csr;
+        if (!c) return src;
+        RowLists r = src;
+        for (auto &z : r.val) z = std::conj(z);
+        return r;
+    }
+};
+
+static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
+    const int64_t m = X.nr;
+    const int nslices = (int)((m + 63) / 64);
+    std::vector<int32_t> off(nslices + 1, 0);
+    for (int k = 0; k < nslices; ++k) {
+        int w = 0;
+        for (int64_t r = (int64_t)k * 64; r < std::min<int64_t>(m, (int64_t)k * 64 + 64); ++r)
+            w = std::max(w, X.ptr[r + 1] - X.ptr[r]);
+        off[k + 1] = off[k] + w * 64;
+    }
+    std::vector<int32_t> col(off[nslices], -1);
+    std::vector<cd> val(off[nslices], cd(0, 0));
+    for (int k = 0; k < nslices; ++k) {
+        for (int64_t r = (int64_t)k * 64; r < std::min<int64_t>(m, (int64_t)k * 64 + 64); ++r) {
+            int p = 0;
+            for (int32_t q = X.ptr[r]; q < X.ptr[r + 1]; ++q, ++p) {
+                const int64_t e = off[k] + (int64_t)p * 64 + (r - (int64_t)k * 64);
+                col[e] = X.idx[q];
+                val[e] = X.val[q];
+            }
+        }
+    }
+    QOB_TRY(S.off.upload(off.data(), off.size() * 4));
+    QOB_TRY(S.col.upload(col.data(), col.size() * 4));
+    QOB_TRY(upload_vals(S.val, val, dt));
+    S.dev.m = m;
+    S.dev.n = X.nc;
+    S.dev.nslices = nslices;
+    S.dev.slice_off = (const int32_t *)S.off.p;
+    S.dev.col = (const int32_t *)S.col.p;
+    S.dev.val = S.val.p;
+    return QOB_OK;
+}
+
+static qob_status build_csc(const RowLists &Xcols, int64_t xrows, qob_dtype dt, CscTab &C) {
+    // Xcols: columns of X as lists (i.e. rows of X^T)
+    QOB_TRY(C.ptr.upload(Xcols.ptr.data(), Xcols.ptr.size() * 4));
+    QOB_TRY(C.row.upload(Xcols.idx.data(), Xcols.idx.size() * 4));
+    QOB_TRY(upload_vals(C.val, Xcols.val, dt));
+    C.dev.m = xrows;
+    C.dev.n = Xcols.nr;
+    C.dev.colptr = (const int32_t *)C.ptr.p;
+    C.dev.row = (const int32_t *)C.row.p;
+    C.dev.val = C.val.p;
+    return QOB_OK;
+}
+
+static qob_status sparse_sell(SparseData &S, int v, qob_dtype dt, SellDev &out) {
+    std::lock_guard<std::mutex> g(S.mu);
+    const int key = v * 2 + (int)dt;
+    auto it = S.sell.find(key);
+    if (it == S.sell.end()) {
+        auto t = std::make_shared<SellTab>();
+        QOB_TRY(build_sell(S.rows(v), dt, *t));
+        it = S.sell.emplace(key, t).first;
+    }
+    out = it->second->dev;
+    return QOB_OK;
+}
+
+static qob_status sparse_csc(SparseData &S, int v, qob_dtype dt, CscDev &out) {
+    std::lock_guard<std::mutex> g(S.mu);
+    const int key = v * 2 + (int)dt;
+    auto it = S.cols.find(key);
+    if (it == S.cols.end()) {
+        auto t = std::make_shared<CscTab>();
+        // columns of op_v(M) = rows of op_v(M)^T = rows of op_{v^T}(M)
+        const RowLists cl = S.rows(v ^ V_T);
+        const int64_t xr = (v & 1) ? S.n : S.m;
+        QOB_TRY(build_csc(cl, xr, dt, *t));
+        it = S.cols.emplace(key, t).first;
+    }
+    out = it->second->dev;
+    return QOB_OK;
+}
+
+// ---------------------------------------------------------------- dense op
+struct DenseData {
+    int64_t m = 0, n = 0;
+    std::vector<cd> a;  // column-major m x n
+    std::mutex mu;
+    std::map<int, std::shared_ptr<DevBuf>> dev;  // key var*2 + dt
+    // op_v(M) as a column-major host matrix (rows x cols)
+    std::vector<cd> mat(int v, int64_t &rows, int64_t &cols) const {
+        const bool t = v & 1, c = (v >> 1) & 1;
+        rows = t ? n : m;
+        cols = t ? m : n;
+        std::vector<cd> r((size_t)(m * n));
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                cd z = a[i + j * m];
+                if (c) z = std::conj(z);
+                if (t) r[j + i * n] = z;
+                else r[i + j * m] = z;
+            }
+        return r;
+    }
+    RowLists rows_of(int v) const {
+        int64_t r, c;
+        std::vector<cd> M = mat(v, r, c);
+        RowLists R;
+        R.nr = r;
+        R.nc = c;
+        R.ptr.assign(r + 1, 0);
+        for (int64_t i = 0; i < r; ++i) {
+            for (int64_t j = 0; j < c; ++j) {
+                R.idx.push_back((int32_t)j);
+                R.val.push_back(M[i + j * r]);
+            }
+            R.ptr[i + 1] = (int32_t)R.idx.size();
+        }
+        return R;
+    }
+};
+
+static qob_status dense_dev(DenseData &D, int v, qob_dtype dt, const void *&ptr) {
+    std::lock_guard<std::mutex> g(D.mu);
+    const int key = v * 2 + (int)dt;
+    auto it = D.dev.find(key);
+    if (it == D.dev.end()) {
+        int64_t r, c;
+        auto b = std::make_shared<DevBuf>();
+        QOB_TRY(upload_vals(*b, D.mat(v, r, c), dt));
+        it = D.dev.emplace(key, b).first;
+    }
+    ptr = it->second->p;
+    return QOB_OK;
+}
+
+// ---------------------------------------------------------------- op nodes
+enum Kind { K_SPARSE, K_DENSE, K_EYE, K_LT, K_LS };
+
+struct GatherPlan {
+    DevBuf prog, rowptr, koff, vals, site_info;
+    GatherPlanDev dev;
+};
+
+struct OpNode {
+    Kind kind = K_EYE;
+    int64_t m = 0, n = 0;
+    int var = V_N;
+    std::shared_ptr<SparseData> sp;
+    std::shared_ptr<DenseData> de;
+    // LazyTensor
+    std::vector<int64_t> shl, shr;
+    cd factor = cd(1, 0);
+    std::vector<int> fsite;  // 0-based, sorted
+    std::vector<std::shared_ptr<OpNode>> fop;
+    // LazySum
+    std::vector<cd> coef;
+    std::vector<std::shared_ptr<OpNode>> terms;
+    // plan cache
+    std::mutex mu;
+    std::map<std::string, std::shared_ptr<GatherPlan>> gplans;
+    std::map<std::string, std::shared_ptr<SparseData>> sums;
+    uint64_t version = 0;  // bumped when coefficients change
+};
+
+static bool is_pure_sparse_factor(const OpNode &o) {
+    return (o.kind == K_SPARSE && o.var == V_N) || o.kind == K_EYE;
+}
+// _is_pure_sparse (src/operators_lazytensor.jl:520) with the non-empty
+// condition of :542.
+static bool lt_pure_sparse(const OpNode &o) {
+    if (o.kind != K_LT || o.fop.empty()) return false;
+    for (auto &f : o.fop)
+        if (!is_pure_sparse_factor(*f)) return false;
+    return true;
+}
+
+}  // namespace qob
+
+struct qob_op_s {
+    std::shared_ptr<qob::OpNode> node;
+};
+
+namespace qob {
+
+// ------------------------------------------------------------- workspace
+struct Workspace {
+    DevBuf slot[5];
+};
+
+}  // namespace qob
+
+struct qob_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    qob_mode mode = QOB_MODE_FUSED;
+    std::mutex mu;
+    std::unordered_map<hipStream_t, std::unique_ptr<qob::Workspace>> ws;
+    rocblas_handle blas = nullptr;
+    std::mutex blas_mu;
+};
+
+namespace qob {
+
+enum { WS_TMP1 = 0, WS_TMP2 = 1, WS_RED = 2, WS_SCAL = 3, WS_EXP = 4, WS_NSLOT = 5 };
+
+static qob_status ws_get(qob_ctx ctx, hipStream_t s, int slot, size_t bytes, void *&p) {
+    Workspace *w;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto &u = ctx->ws[s];
+        if (!u) u.reset(new Workspace());
+        w = u.get();
+    }
+    if (w->slot[slot].bytes < bytes) {
+        // growing frees the old buffer; queued work on s may still read it
+        QOB_HIP(hipStreamSynchronize(s));
+        QOB_TRY(w->slot[slot].alloc(bytes));
+    }
+    p = w->slot[slot].p;
+    return QOB_OK;
+}
+
+static hipStream_t strm(qob_ctx ctx, qob_stream s) { return s ? (hipStream_t)s : ctx->stream; }
+
+// ------------------------------------------------------------ plan building
+struct TermSpec {
+    cd coef;
+    std::vector<int> sites;                // sorted
+    std::vector<const RowLists *> X;       // X[i]: out_d x in_d matrix at sites[i]
+};
+
+struct HostPlan {
+    std::vector<int32_t> prog, rowptr, koff, site_info;
+    std::vector<cd> vals;
+    int nterms = 0, nsites = 0;
+    bool square = true;
+};
+
+static int64_t prod(const std::vector<int64_t> &v, size_t a, size_t b) {
+    int64_t p = 1;
+    for (size_t i = a; i < b; ++i) p *= v[i];
+    return p;
+}
+
+// Compile a sum of tensor-product terms into gather tables.
+//   out_shape / in_shape : per-site dims of the output and input spaces
+//   A term with factor X_i (out_d x in_d) at site s_i acts as the identity
+//   (truncated to min(out_d, in_d)) on every other site.
+static qob_status compile_plan(const std::vector<int64_t> &out_shape, const std::vector<int64_t> &in_shape,
+                               const std::vector<TermSpec> &terms, HostPlan &H) {
+    const int N = (int)out_shape.size();
+    H.square = (out_shape == in_shape);
+    H.nsites = N;
+    const int64_t Dout = prod(out_shape, 0, N), Din = prod(in_shape, 0, N);
+    if (Dout >= (1ll << 31) || Din >= (1ll << 31))
+        return fail(QOB_EUNSUPPORTED, "lazy gather plan needs dimensions below 2^31");
+    if (!H.square && N > 32) return fail(QOB_EUNSUPPORTED, "non-square lazy operators with more than 32 sites");
+    std::vector<int64_t> so(N), si(N);
+    for (int s = 0; s < N; ++s) {
+        so[s] = prod(out_shape, 0, s);
+        si[s] = prod(in_shape, 0, s);
+    }
+    auto site_rec = [&](int s, int32_t lstride, int32_t tbl) {
+        FastDiv a = make_fastdiv((uint32_t)so[s]), b = make_fastdiv((uint32_t)out_shape[s]);
+        return std::vector<int32_t>{(int32_t)so[s], (int32_t)a.m, (int32_t)a.l, (int32_t)out_shape[s],
+                                    (int32_t)b.m, (int32_t)b.l, lstride, tbl, (int32_t)si[s]};
+    };
+    if (!H.square) {
+        for (int s = 0; s < N; ++s) {
+            FastDiv a = make_fastdiv((uint32_t)so[s]), b = make_fastdiv((uint32_t)out_shape[s]);
+            int32_t rec[8] = {(int32_t)so[s], (int32_t)a.m, (int32_t)a.l, (int32_t)out_shape[s],
+                              (int32_t)b.m, (int32_t)b.l, (int32_t)si[s], (int32_t)in_shape[s]};
+            H.site_info.insert(H.site_info.end(), rec, rec + 8);
+        }
+    }
+
+    // Group terms by site set (expanded form); keep product-form terms apart.
+    struct Group {
+        std::vector<int> sites;
+        std::vector<std::map<int64_t, cd>> rows;  // per local row: koff -> value
+    };
+    std::vector<Group> groups;
+    std::map<std::vector<int>, size_t> gidx;
+    struct Prod {
+        const TermSpec *t;
+    };
+    std::vector<Prod> prods;
+
+    for (const TermSpec &t : terms) {
+        const int q = (int)t.sites.size();
+        int64_t nloc = 1;
+        bool mono = true;
+        for (int i = 0; i < q; ++i) {
+            const int s = t.sites[i];
+            if (t.X[i]->nr != out_shape[s] || t.X[i]->nc != in_shape[s])
+                return fail(QOB_EDIM, "factor at site %d is %lldx%lld, site dims are %lldx%lld", s + 1,
+                            (long long)t.X[i]->nr, (long long)t.X[i]->nc, (long long)out_shape[s],
+                            (long long)in_shape[s]);
+            nloc *= out_shape[s];
+            mono = mono && t.X[i]->monomial();
+        }
+        if (mono && nloc > 4096) {
+            prods.push_back({&t});
+            continue;
+        }
+        if (nloc > (1ll << 22)) return fail(QOB_EUNSUPPORTED, "LazyTensor term with local dimension %lld", (long long)nloc);
+        auto it = gidx.find(t.sites);
+        if (it == gidx.end()) {
+            Group g;
+            g.sites = t.sites;
+            g.rows.resize((size_t)nloc);
+            it = gidx.emplace(t.sites, groups.size()).first;
+            groups.push_back(std::move(g));
+        }
+        Group &g = groups[it->second];
+        // odometer over the local output digits (first site fastest)
+        std::vector<int64_t> jd(q, 0);
+        for (int64_t jloc = 0; jloc < nloc; ++jloc) {
+            // Cartesian product over the factors' row entries
+            std::vector<int32_t> pp(q), pe(q);
+            bool empty = false;
+            for (int i = 0; i < q; ++i) {
+                pp[i] = t.X[i]->ptr[jd[i]];
+                pe[i] = t.X[i]->ptr[jd[i] + 1];
+                if (pp[i] == pe[i]) empty = true;
+            }
+            if (!empty) {
+                std::vector<int32_t> cur = pp;
+                while (true) {
+                    cd v = t.coef;
+                    int64_t koff = 0;
+                    for (int i = 0; i < q; ++i) {
+                        const int s = t.sites[i];
+                        const int64_t k = t.X[i]->idx[cur[i]];
+                        v *= t.X[i]->val[cur[i]];
+                        koff += H.square ? (k - jd[i]) * so[s] : k * si[s];
+                    }
+                    g.rows[jloc][koff] += v;
+                    int i = 0;
+                    while (i < q) {
+                        if (++cur[i] < pe[i]) break;
+                        cur[i] = pp[i];
+                        ++i;
+                    }
+                    if (i == q) break;
+                }
+            }
+            for (int i = 0; i < q; ++i) {
+                if (++jd[i] < out_shape[t.sites[i]]) break;
+                jd[i] = 0;
+            }
+        }
+    }
+
+    // Emit expanded groups (diagonal ones first so their loads merge).
+    std::stable_sort(groups.begin(), groups.end(), [&](const Group &a, const Group &b) {
+        auto diag = [&](const Group &g) {
+            if (!H.square) return false;
+            for (auto &r : g.rows)
+                for (auto &kv : r)
+                    if (kv.first != 0) return false;
+            return true;
+        };
+        return diag(a) && !diag(b);
+    });
+    for (const Group &g : groups) {
+        bool diag = H.square;
+        for (auto &r : g.rows)
+            for (auto &kv : r)
+                if (kv.first != 0) diag = false;
+        const int q = (int)g.sites.size();
+        uint32_t fmask = 0;
+        for (int s : g.sites) fmask |= (s < 32) ? (1u << s) : 0u;
+        const int32_t base = (int32_t)H.rowptr.size();
+        H.prog.insert(H.prog.end(), {0, diag ? 1 : 0, q, base, (int32_t)fmask});
+        int32_t ls = 1;
+        for (int i = 0; i < q; ++i) {
+            auto rec = site_rec(g.sites[i], ls, 0);
+            H.prog.insert(H.prog.end(), rec.begin(), rec.end());
+            ls *= (int32_t)out_shape[g.sites[i]];
+        }
+        for (auto &r : g.rows) {
+            H.rowptr.push_back((int32_t)H.koff.size());
+            for (auto &kv : r) {
+                H.koff.push_back((int32_t)kv.first);
+                H.vals.push_back(kv.second);
+            }
+        }
+        H.rowptr.push_back((int32_t)H.koff.size());
+        H.nterms++;
+    }
+    // Emit product-form terms: per site a table over the output digit.
+    for (const Prod &pr : prods) {
+        const TermSpec &t = *pr.t;
+        const int q = (int)t.sites.size();
+        uint32_t fmask = 0;
+        for (int s : t.sites) fmask |= (s < 32) ? (1u << s) : 0u;
+        bool diag = H.square;
+        for (int i = 0; i < q; ++i)
+            for (int64_t j = 0; j < t.X[i]->nr; ++j)
+                for (int32_t p = t.X[i]->ptr[j]; p < t.X[i]->ptr[j + 1]; ++p)
+                    if (t.X[i]->idx[p] != j) diag = false;
+        H.prog.insert(H.prog.end(), {1, diag ? 1 : 0, q, 0, (int32_t)fmask});
+        for (int i = 0; i < q; ++i) {
+            const int s = t.sites[i];
+            const int32_t tbl = (int32_t)H.koff.size();
+            auto rec = site_rec(s, 0, tbl);
+            H.prog.insert(H.prog.end(), rec.begin(), rec.end());
+            for (int64_t j = 0; j < out_shape[s]; ++j) {
+                const RowLists &X = *t.X[i];
+                if (X.ptr[j + 1] > X.ptr[j]) {
+                    const int64_t k = X.idx[X.ptr[j]];
+                    H.koff.push_back((int32_t)(H.square ? (k - j) * so[s] : k * si[s]));
+                    cd v = X.val[X.ptr[j]];
+                    if (i == 0) v *= t.coef;
+                    H.vals.push_back(v);
+                } else {
+                    H.koff.push_back(0);
+                    H.vals.push_back(cd(0, 0));
+                }
+            }
+        }
+        H.nterms++;
+    }
+    if (H.rowptr.empty()) H.rowptr.push_back(0);
+    if (H.koff.empty()) {
+        H.koff.push_back(0);
+        H.vals.push_back(cd(0, 0));
+    }
+    return QOB_OK;
+}
+
+static qob_status upload_plan(const HostPlan &H, qob_dtype dt, GatherPlan &P) {
+    QOB_TRY(P.prog.upload(H.prog.data(), std::max<size_t>(H.prog.size(), 1) * 4));
+    QOB_TRY(P.rowptr.upload(H.rowptr.data(), H.rowptr.size() * 4));
+    QOB_TRY(P.koff.upload(H.koff.data(), H.koff.size() * 4));
+    QOB_TRY(upload_vals(P.vals, H.vals, dt));
+    if (!H.site_info.empty()) QOB_TRY(P.site_info.upload(H.site_info.data(), H.site_info.size() * 4));
+    P.dev.prog = (const int32_t *)P.prog.p;
+    P.dev.nterms = H.nterms;
+    P.dev.rowptr = (const int32_t *)P.rowptr.p;
+    P.dev.koff = (const int32_t *)P.koff.p;
+    P.dev.vals = P.vals.p;
+    P.dev.nsites = H.nsites;
+    P.dev.site_info = (const int32_t *)P.site_info.p;
+    P.dev.square = H.square;
+    return QOB_OK;
+}
+
+// Rows of a pure-sparse factor matrix for the gather.  trans: use the
+// transpose (right application / Bra).
+static RowLists factor_rows(const OpNode &f, bool trans) {
+    if (f.kind == K_SPARSE) return f.sp->rows(f.var ^ (trans ? V_T : V_N));
+    if (f.kind == K_DENSE) return f.de->rows_of(f.var ^ (trans ? V_T : V_N));
+    return trans ? identity_rows(f.n, f.m) : identity_rows(f.m, f.n);
+}
+
+// Gather plan for a set of pure-sparse LazyTensors (one term per tensor).
+// trans = false: acts on a Ket / rows of rho (out space = basis_l);
+// trans = true : transposed, acts on a Bra / columns of rho (out = basis_r).
+static qob_status lazy_plan(OpNode &owner, const std::string &key, const std::vector<OpNode *> &lts,
+                            const std::vector<cd> &coefs, bool trans, qob_dtype dt, GatherPlanDev &out) {
+    std::lock_guard<std::mutex> g(owner.mu);
+    const std::string k = key + (trans ? ":T:" : ":N:") + std::to_string((int)dt);
+    auto it = owner.gplans.find(k);
+    if (it == owner.gplans.end()) {
+        const OpNode &first = *lts[0];
+        const std::vector<int64_t> &out_shape = trans ? first.shr : first.shl;
+        const std::vector<int64_t> &in_shape = trans ? first.shl : first.shr;
+        std::vector<std::vector<RowLists>> store(lts.size());
+        std::vector<TermSpec> specs;
+        for (size_t t = 0; t < lts.size(); ++t) {
+            const OpNode &lt = *lts[t];
+            TermSpec ts;
+            ts.coef = coefs[t] * lt.factor;
+            store[t].reserve(lt.fop.size());
+            for (size_t i = 0; i < lt.fop.size(); ++i) {
+                if (lt.fop[i]->kind == K_EYE) continue;  // identity site (truncating)
+                store[t].push_back(factor_rows(*lt.fop[i], trans));
+            }
+            size_t k2 = 0;
+            for (size_t i = 0; i < lt.fop.size(); ++i) {
+                if (lt.fop[i]->kind == K_EYE) continue;
+                ts.sites.push_back(lt.fsite[i]);
+                ts.X.push_back(&store[t][k2++]);
+            }
+            specs.push_back(std::move(ts));
+        }
+        HostPlan H;
+        QOB_TRY(compile_plan(out_shape, in_shape, specs, H));
+        auto P = std::make_shared<GatherPlan>();
+        QOB_TRY(upload_plan(H, dt, *P));
+        it = owner.gplans.emplace(k, P).first;
+    }
+    out = it->second->dev;
+    return QOB_OK;
+}
+
+// ------------------------------------------------------------------ rocBLAS
+static qob_status blas_handle(qob_ctx ctx, rocblas_handle &h) {
+    if (!ctx->blas) {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+            return fail(QOB_EHIP, "rocblas_create_handle failed");
+    }
+    h = ctx->blas;
+    return QOB_OK;
+}
+
+// C (m x n) = alpha * op(A) * op(B) + beta * C, column-major, contiguous
+// leading dimensions lda/ldb/ldc, optional strided batch.
+static qob_status blas_gemm(qob_ctx ctx, hipStream_t s, qob_dtype dt, char ta, char tb, int64_t m, int64_t n,
+                            int64_t k, cd alpha, const void *A, int64_t lda, int64_t sa, const void *B,
+                            int64_t ldb, int64_t sb, cd beta, void *C, int64_t ldc, int64_t sc, int64_t batch) {
+    if (m == 0 || n == 0) return QOB_OK;
+    std::lock_guard<std::mutex> g(ctx->blas_mu);
+    rocblas_handle h;
+    QOB_TRY(blas_handle(ctx, h));
+    rocblas_set_stream(h, s);
+    auto op = [](char c) {
+        return c == 'N' ? rocblas_operation_none
+                        : (c == 'T' ? rocblas_operation_transpose : rocblas_operation_conjugate_transpose);
+    };
+    rocblas_status st;
+    if (k == 0) {
+        // no product: C = beta * C
+        if (is_zero(beta)) {
+            for (int64_t b = 0; b < batch; ++b)
+                if (hipMemsetAsync((char *)C + b * sc * esize(dt), 0, (size_t)(ldc * n) * esize(dt), s) != hipSuccess)
+                    return fail(QOB_EHIP, "hipMemsetAsync failed");
+            return QOB_OK;
+        }
+        for (int64_t b = 0; b < batch; ++b)
+            if (launch_scale(dt, ldc * n, d2(beta), (char *)C + b * sc * esize(dt), s) != hipSuccess)
+                return fail(QOB_EHIP, "scale launch failed");
+        return QOB_OK;
+    }
+    if (dt == QOB_C64) {
+        rocblas_double_complex a(alpha.real(), alpha.imag()), b(beta.real(), beta.imag());
+        st = rocblas_zgemm_strided_batched(h, op(ta), op(tb), m, n, k, &a, (const rocblas_double_complex *)A, lda, sa,
+                                           (const rocblas_double_complex *)B, ldb, sb, &b,
+                                           (rocblas_double_complex *)C, ldc, sc, batch);
+    } else {
+        rocblas_float_complex a((float)alpha.real(), (float)alpha.imag()), b((float)beta.real(), (float)beta.imag());
+        st = rocblas_cgemm_strided_batched(h, op(ta), op(tb), m, n, k, &a, (const rocblas_float_complex *)A, lda, sa,
+                                           (const rocblas_float_complex *)B, ldb, sb, &b,
+                                           (rocblas_float_complex *)C, ldc, sc, batch);
+    }
+    if (st != rocblas_status_success) return fail(QOB_EHIP, "rocBLAS gemm failed (%d)", (int)st);
+    return QOB_OK;
+}
+
+// ------------------------------------------------------------ apply engine
+// A "view" of a dense state for one application: data of size lead x D x
+// trail where the operator acts on the middle index D (decomposed by the
+// operator's site shape).  Ket: lead = trail = 1.  rho left: lead = 1,
+// trail = columns.  rho right: lead = rows, trail = 1.  Bra: as a Ket of the
+// transposed operator.
+struct Apply {
+    qob_ctx ctx;
+    hipStream_t s;
+    qob_dtype dt;
+    int64_t lead, trail;  // one of them is 1
+    bool trans;           // use transposed factors (right / Bra)
+};
+
+static qob_status apply_op(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta);
+
+static int64_t op_out_dim(const OpNode &op, bool trans) { return trans ? op.n : op.m; }
+static int64_t op_in_dim(const OpNode &op, bool trans) { return trans ? op.m : op.n; }
+
+// out = alpha * in (padded/truncated) + beta * out over the lead/trail view
+static qob_status apply_embed(const Apply &ap, int64_t dout, int64_t din, const void *in, void *out, cd alpha,
+                              cd beta) {
+    hipError_t e;
+    if (ap.lead == 1) e = launch_embed(ap.dt, in, din, ap.trail, out, dout, ap.trail, d2(alpha), d2(beta), ap.s);
+    else e = launch_embed(ap.dt, in, ap.lead, din, out, ap.lead, dout, d2(alpha), d2(beta), ap.s);
+    if (e != hipSuccess) return fail(QOB_EHIP, "embed launch: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+static qob_status apply_scale(const Apply &ap, int64_t dout, void *out, cd beta) {
+    hipError_t e = launch_scale(ap.dt, dout * ap.lead * ap.trail, d2(beta), out, ap.s);
+    if (e != hipSuccess) return fail(QOB_EHIP, "scale launch: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+// Run a compiled gather plan on the view.
+static qob_status apply_gather(const Apply &ap, const GatherPlanDev &P, int64_t din, int64_t dout, const void *in,
+                               void *out, cd alpha, cd beta) {
+    hipError_t e;
+    if (ap.lead == 1) e = launch_gather_rows(ap.dt, P, in, din, out, dout, ap.trail, d2(alpha), d2(beta), ap.s);
+    else e = launch_gather_cols(ap.dt, P, in, din, out, dout, ap.lead, d2(alpha), d2(beta), ap.s);
+    if (e != hipSuccess) return fail(QOB_EHIP, "gather launch: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+// SparseOperator (src/operators_sparse.jl:199-202 -> src/sparsematrix.jl)
+static qob_status apply_sparse(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
+    const int v = op.var ^ (ap.trans ? V_T : V_N);
+    if (ap.lead == 1) {
+        SellDev X;
+        QOB_TRY(sparse_sell(*op.sp, v, ap.dt, X));
+        hipError_t e = launch_sell_left(ap.dt, X, in, ap.trail, out, d2(alpha), d2(beta), ap.s);
+        if (e != hipSuccess) return fail(QOB_EHIP, "sparse left launch: %s", hipGetErrorString(e));
+    } else {
+        // right application B * op: columns of op_v^T ... the view applies
+        // X = op_v along the column index: out[:, c] = sum_r X[c, r] in[:, r]
+        // i.e. R = B * X^T  -> CSC of X^T = rows of X
+        CscDev C;
+        QOB_TRY(sparse_csc(*op.sp, v ^ V_T, ap.dt, C));
+        hipError_t e = launch_csc_right(ap.dt, C, in, ap.lead, out, d2(alpha), d2(beta), ap.s);
+        if (e != hipSuccess) return fail(QOB_EHIP, "sparse right launch: %s", hipGetErrorString(e));
+    }
+    return QOB_OK;
+}
+
+// DenseOperator (src/operators_dense.jl:394-396, BLAS).  The device copy of
+// the node's own matrix (M x N) is used with BLAS transposition flags.
+static qob_status apply_dense(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
+    const void *A;
+    QOB_TRY(dense_dev(*op.de, op.var, ap.dt, A));
+    const int64_t M = op.m, N = op.n;
+    if (ap.lead == 1) {
+        if (!ap.trans)  // out (M x trail) = A in
+            return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'N', M, ap.trail, N, alpha, A, M, 0, in, N, 0, beta, out, M, 0, 1);
+        // Bra: out (N x trail) = A^T in  (src/operators_dense.jl:396)
+        return blas_gemm(ap.ctx, ap.s, ap.dt, 'T', 'N', N, ap.trail, M, alpha, A, M, 0, in, M, 0, beta, out, N, 0, 1);
+    }
+    if (ap.trans)  // rho right: out (lead x N) = in (lead x M) A
+        return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'N', ap.lead, N, M, alpha, in, ap.lead, 0, A, M, 0, beta, out,
+                         ap.lead, 0, 1);
+    return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'T', ap.lead, M, N, alpha, in, ap.lead, 0, A, M, 0, beta, out, ap.lead,
+                     0, 1);
+}
+
+// One step of the tensor-product chain: mode product on site k of the
+// current shape (between lead and trail).
+struct ChainOp {
+    int site;
+    OpNode *f;            // factor node (dense / sparse / eye)
+    int64_t d_out, d_in;  // effective dims of the (possibly transposed) factor
+};
+
+static qob_status chain_step(const Apply &ap, OpNode &owner, const std::string &key, const ChainOp &c,
+                             const std::vector<int64_t> &shape_in, const void *in, void *out, cd alpha, cd beta) {
+    const int N = (int)shape_in.size();
+    int64_t n1 = ap.lead * prod(shape_in, 0, c.site);
+    int64_t n3 = prod(shape_in, c.site + 1, N) * ap.trail;
+    OpNode &f = *c.f;
+    if (f.kind == K_DENSE && c.d_in <= 64) {
+        const void *A;
+        QOB_TRY(dense_dev(*f.de, f.var ^ (ap.trans ? V_T : V_N), ap.dt, A));
+        hipError_t e = launch_modek(ap.dt, A, c.d_out, c.d_in, in, out, n1, n3, d2(alpha), d2(beta), ap.s);
+        if (e != hipSuccess) return fail(QOB_EHIP, "mode-k launch: %s", hipGetErrorString(e));
+        return QOB_OK;
+    }
+    if (f.kind == K_DENSE) {
+        const void *A;
+        QOB_TRY(dense_dev(*f.de, f.var ^ (ap.trans ? V_T : V_N), ap.dt, A));  // d_out x d_in
+        if (n1 == 1)
+            return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'N', c.d_out, n3, c.d_in, alpha, A, c.d_out, 0, in, c.d_in, 0,
+                             beta, out, c.d_out, 0, 1);
+        return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'T', n1, c.d_out, c.d_in, alpha, in, n1, n1 * c.d_in, A, c.d_out,
+                         0, beta, out, n1, n1 * c.d_out, n3);
+    }
+    // sparse factor or isometry: one-term gather over the full view
+    std::vector<int64_t> full_in, full_out;
+    full_in.push_back(n1);
+    full_in.push_back(c.d_in);
+    full_in.push_back(n3);
+    full_out = full_in;
+    full_out[1] = c.d_out;
+    GatherPlanDev P;
+    {
+        std::lock_guard<std::mutex> g(owner.mu);
+        char kb[160];
+        snprintf(kb, sizeof kb, "%s:chain:%d:%lld:%lld:%lld:%lld:%d:%d", key.c_str(), c.site, (long long)n1,
+                 (long long)c.d_in, (long long)c.d_out, (long long)n3, (int)ap.trans, (int)ap.dt);
+        auto it = owner.gplans.find(kb);
+        if (it == owner.gplans.end()) {
+            RowLists X = factor_rows(f, ap.trans);
+            TermSpec ts;
+            ts.coef = cd(1, 0);
+            ts.sites = {1};
+            ts.X = {&X};
+            HostPlan H;
+            QOB_TRY(compile_plan(full_out, full_in, {ts}, H));
+            auto GP = std::make_shared<GatherPlan>();
+            QOB_TRY(upload_plan(H, ap.dt, *GP));
+            it = owner.gplans.emplace(kb, GP).first;
+        }
+        P = it->second->dev;
+    }
+    hipError_t e = launch_gather_rows(ap.dt, P, in, n1 * c.d_in * n3, out, n1 * c.d_out * n3, 1, d2(alpha), d2(beta),
+                                      ap.s);
+    if (e != hipSuccess) return fail(QOB_EHIP, "gather launch: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+// LazyTensor through the tensor-product chain: _tp_sum_matmul!
+// (src/operators_lazytensor.jl:443-488) with the operator list of
+// _tpops_tuple (:523-535, square identities dropped) followed by the
+// explicit isometries of _explicit_isometries (:491-514).
+static qob_status apply_lt_chain(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
+    const std::vector<int64_t> &sh_out = ap.trans ? op.shr : op.shl;
+    const std::vector<int64_t> &sh_in = ap.trans ? op.shl : op.shr;
+    const int N = (int)sh_in.size();
+    std::vector<ChainOp> ops;
+    std::vector<bool> used(N, false);
+    for (size_t i = 0; i < op.fop.size(); ++i) {
+        OpNode &f = *op.fop[i];
+        used[op.fsite[i]] = true;
+        if (f.kind == K_EYE && f.m == f.n) continue;  // square Eye: identity
+        const int64_t dout = ap.trans ? f.n : f.m, din = ap.trans ? f.m : f.n;
+        ops.push_back({op.fsite[i], &f, dout, din});
+    }
+    // isometries for sites whose dims differ and that carry no factor
+    std::vector<std::shared_ptr<OpNode>> isos;
+    for (int s = 0; s < N; ++s) {
+        if (sh_out[s] != sh_in[s] && !used[s]) {
+            auto e = std::make_shared<OpNode>();
+            e->kind = K_EYE;
+            e->m = ap.trans ? sh_in[s] : sh_out[s];
+            e->n = ap.trans ? sh_out[s] : sh_in[s];
+            isos.push_back(e);
+            ops.push_back({s, e.get(), sh_out[s], sh_in[s]});
+        }
+    }
+    const cd a = alpha * op.factor;
+    const int64_t Din = prod(sh_in, 0, N), Dout = prod(sh_out, 0, N);
+    if (ops.empty()) {
+        // n_ops == 0: result = alpha * b + beta * result (:453-458)
+        if (Din != Dout) return fail(QOB_EDIM, "LazyTensor without factors on unequal dimensions");
+        return apply_embed(ap, Dout, Din, in, out, a, beta);
+    }
+    // shapes after each step
+    std::vector<int64_t> cur = sh_in;
+    if (ops.size() == 1) return chain_step(ap, op, "lt", ops[0], cur, in, out, a, beta);
+    // temporaries sized for the largest intermediate
+    size_t maxel = 0;
+    {
+        std::vector<int64_t> c2 = sh_in;
+        for (size_t i = 0; i + 1 < ops.size(); ++i) {
+            c2[ops[i].site] = ops[i].d_out;
+            maxel = std::max<size_t>(maxel, (size_t)(prod(c2, 0, N) * ap.lead * ap.trail));
+        }
+    }
+    void *t1, *t2;
+    QOB_TRY(ws_get(ap.ctx, ap.s, WS_TMP1, maxel * esize(ap.dt), t1));
+    if (ops.size() > 2) QOB_TRY(ws_get(ap.ctx, ap.s, WS_TMP2, maxel * esize(ap.dt), t2));
+    else t2 = nullptr;
+    QOB_TRY(chain_step(ap, op, "lt", ops[0], cur, in, t1, a, cd(0, 0)));
+    cur[ops[0].site] = ops[0].d_out;
+    for (size_t i = 1; i + 1 < ops.size(); ++i) {
+        QOB_TRY(chain_step(ap, op, "lt", ops[i], cur, t1, t2, cd(1, 0), cd(0, 0)));
+        cur[ops[i].site] = ops[i].d_out;
+        std::swap(t1, t2);
+    }
+    return chain_step(ap, op, "lt", ops.back(), cur, t1, out, cd(1, 0), beta);
+}
+
+static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
+    const int64_t dout = op_out_dim(op, ap.trans);
+    if (is_zero(alpha)) return apply_scale(ap, dout, out, beta);  // :540
+    if (lt_pure_sparse(op)) {
+        // _mul_puresparse! -> _gemm_puresparse (:729-751)
+        GatherPlanDev P;
+        std::vector<OpNode *> one{&op};
+        QOB_TRY(lazy_plan(op, "lt", one, {cd(1, 0)}, ap.trans, ap.dt, P));
+        return apply_gather(ap, P, op_in_dim(op, ap.trans), dout, in, out, alpha, beta);
+    }
+    return apply_lt_chain(ap, op, in, out, alpha, beta);
+}
+
+// LazySum (src/operators_lazysum.jl:189-238).  Fused mode groups the terms:
+// all pure-sparse LazyTensors become one gather launch, all SparseOperators
+// are summed into one sparse matrix; the remaining terms follow with beta=1.
+static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
+    const int64_t dout = op_out_dim(op, ap.trans);
+    if (op.terms.empty() || is_zero(alpha)) return apply_scale(ap, dout, out, beta);
+    if (ap.ctx->mode == QOB_MODE_PLAIN) {
+        for (size_t t = 0; t < op.terms.size(); ++t)
+            QOB_TRY(apply_op(ap, *op.terms[t], in, out, alpha * op.coef[t], t == 0 ? beta : cd(1, 0)));
+        return QOB_OK;
+    }
+    std::vector<OpNode *> lts;
+    std::vector<cd> lcoef;
+    std::vector<size_t> sp_terms, rest;
+    for (size_t t = 0; t < op.terms.size(); ++t) {
+        OpNode &x = *op.terms[t];
+        if (lt_pure_sparse(x)) {
+            lts.push_back(&x);
+            lcoef.push_back(op.coef[t]);
+        } else if (x.kind == K_SPARSE) {
+            sp_terms.push_back(t);
+        } else {
+            rest.push_back(t);
+        }
+    }
+    cd b = beta;
+    if (!lts.empty()) {
+        GatherPlanDev P;
+        QOB_TRY(lazy_plan(op, "ls" + std::to_string(op.version), lts, lcoef, ap.trans, ap.dt, P));
+        QOB_TRY(apply_gather(ap, P, op_in_dim(op, ap.trans), dout, in, out, alpha, b));
+        b = cd(1, 0);
+    }
+    if (!sp_terms.empty()) {
+        // summed sparse matrix sum_t c_t op_t(M_t), built once per
+        // coefficient set and cached on the LazySum node
+        std::shared_ptr<SparseData> SD;
+        {
+            std::lock_guard<std::mutex> g(op.mu);
+            const std::string k = "sps" + std::to_string(op.version);
+            auto it = op.sums.find(k);
+            if (it == op.sums.end()) {
+                std::vector<std::map<int32_t, cd>> acc((size_t)op.m);
+                for (size_t t : sp_terms) {
+                    OpNode &x = *op.terms[t];
+                    RowLists R = x.sp->rows(x.var);
+                    for (int64_t i = 0; i < R.nr; ++i)
+                        for (int32_t p = R.ptr[i]; p < R.ptr[i + 1]; ++p) acc[i][R.idx[p]] += op.coef[t] * R.val[p];
+                }
+                auto S = std::make_shared<SparseData>();
+                S->m = op.m;
+                S->n = op.n;
+                S->csr.nr = op.m;
+                S->csr.nc = op.n;
+                S->csr.ptr.assign(op.m + 1, 0);
+                for (int64_t i = 0; i < op.m; ++i) {
+                    for (auto &kv : acc[i]) {
+                        S->csr.idx.push_back(kv.first);
+                        S->csr.val.push_back(kv.second);
+                    }
+                    S->csr.ptr[i + 1] = (int32_t)S->csr.idx.size();
+                }
+                S->csc = transpose_rows(S->csr, false);
+                it = op.sums.emplace(k, S).first;
+            }
+            SD = it->second;
+        }
+        OpNode tmp;
+        tmp.kind = K_SPARSE;
+        tmp.m = op.m;
+        tmp.n = op.n;
+        tmp.sp = SD;
+        QOB_TRY(apply_sparse(ap, tmp, in, out, alpha, b));
+        b = cd(1, 0);
+    }
+    for (size_t t : rest) {
+        QOB_TRY(apply_op(ap, *op.terms[t], in, out, alpha * op.coef[t], b));
+        b = cd(1, 0);
+    }
+    return QOB_OK;
+}
+
+static qob_status apply_op(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
+    switch (op.kind) {
+    case K_SPARSE: return apply_sparse(ap, op, in, out, alpha, beta);
+    case K_DENSE: return apply_dense(ap, op, in, out, alpha, beta);
+    case K_EYE: return apply_embed(ap, op_out_dim(op, ap.trans), op_in_dim(op, ap.trans), in, out, alpha, beta);
+    case K_LT: return apply_lt(ap, op, in, out, alpha, beta);
+    case K_LS: return apply_ls(ap, op, in, out, alpha, beta);
+    }
+    return fail(QOB_EUNSUPPORTED, "unknown operator kind");
+}
+
+}  // namespace qob
+
+using namespace qob;
+
+// ====================================================================== ABI
+extern "C" {
+
+const char *qob_last_error(void) { return g_err.c_str(); }
+const char *qob_version(void) { return "qob_hip 0.1.0 (gfx950)"; }
+
+qob_status qob_ctx_create(int device, qob_ctx *out) {
+    if (!out) return fail(QOB_EARG, "null output pointer");
+    int n = 0;
+    QOB_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(QOB_EARG, "device %d not present (%d devices)", device, n);
+    QOB_HIP(hipSetDevice(device));
+    auto *c = new qob_ctx_s();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(QOB_EHIP, "hipStreamCreateWithFlags failed");
+    }
+    *out = c;
+    return QOB_OK;
+}
+
+qob_status qob_ctx_destroy(qob_ctx ctx) {
+    if (!ctx) return QOB_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ws.clear();
+    if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return QOB_OK;
+}
+
+qob_status qob_ctx_set_mode(qob_ctx ctx, qob_mode mode) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    if (mode != QOB_MODE_FUSED && mode != QOB_MODE_PLAIN) return fail(QOB_EARG, "bad mode %d", (int)mode);
+    ctx->mode = mode;
+    return QOB_OK;
+}
+
+qob_status qob_ctx_get_stream(qob_ctx ctx, qob_stream *out) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    *out = (qob_stream)ctx->stream;
+    return QOB_OK;
+}
+
+qob_status qob_dmalloc(qob_ctx ctx, size_t bytes, void **dptr) {
+    if (!ctx || !dptr) return fail(QOB_EARG, "null argument");
+    if (hipMalloc(dptr, bytes ? bytes : 16) != hipSuccess) {
+        *dptr = nullptr;
+        (void)hipGetLastError();
+        return fail(QOB_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    return QOB_OK;
+}
+
+qob_status qob_dfree(qob_ctx ctx, void *dptr) {
+    (void)ctx;
+    if (dptr) QOB_HIP(hipFree(dptr));
+    return QOB_OK;
+}
+
+qob_status qob_memcpy_htod(qob_ctx ctx, void *dst, const void *src, size_t bytes, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    hipStream_t st = strm(ctx, s);
+    QOB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    QOB_HIP(hipStreamSynchronize(st));
+    return QOB_OK;
+}
+
+qob_status qob_memcpy_dtoh(qob_ctx ctx, void *dst, const void *src, size_t bytes, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    hipStream_t st = strm(ctx, s);
+    QOB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    QOB_HIP(hipStreamSynchronize(st));
+    return QOB_OK;
+}
+
+qob_status qob_memcpy_dtod(qob_ctx ctx, void *dst, const void *src, size_t bytes, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    QOB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, strm(ctx, s)));
+    return QOB_OK;
+}
+
+qob_status qob_stream_create(qob_ctx ctx, qob_stream *out) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    hipStream_t s;
+    QOB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = (qob_stream)s;
+    return QOB_OK;
+}
+
+qob_status qob_stream_destroy(qob_ctx ctx, qob_stream s) {
+    if (!ctx || !s) return fail(QOB_EARG, "null argument");
+    QOB_HIP(hipStreamSynchronize((hipStream_t)s));
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->ws.erase((hipStream_t)s);
+    }
+    QOB_HIP(hipStreamDestroy((hipStream_t)s));
+    return QOB_OK;
+}
+
+qob_status qob_stream_synchronize(qob_ctx ctx, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    QOB_HIP(hipStreamSynchronize(strm(ctx, s)));
+    return QOB_OK;
+}
+
+// ------------------------------------------------------------- operators
+static qob_op wrap(std::shared_ptr<OpNode> n) {
+    auto *h = new qob_op_s();
+    h->node = std::move(n);
+    return h;
+}
+
+qob_status qob_sparse_create(qob_ctx ctx, int64_t m, int64_t n, const int64_t *colptr1, const int64_t *rowval1,
+                             const qob_c64 *nzval, qob_op *out) {
+    if (!ctx || !out || !colptr1) return fail(QOB_EARG, "null argument");
+    if (m < 0 || n < 0 || m >= (1ll << 31) || n >= (1ll << 31)) return fail(QOB_EARG, "bad dims %lldx%lld", (long long)m, (long long)n);
+    const int64_t nnz = colptr1[n] - 1;
+    if (colptr1[0] != 1 || nnz < 0) return fail(QOB_EARG, "colptr must be 1-based");
+    if (nnz > 0 && (!rowval1 || !nzval)) return fail(QOB_EARG, "null rowval/nzval");
+    auto S = std::make_shared<SparseData>();
+    S->m = m;
+    S->n = n;
+    S->csc.nr = n;
+    S->csc.nc = m;
+    S->csc.ptr.resize(n + 1);
+    for (int64_t j = 0; j <= n; ++j) {
+        S->csc.ptr[j] = (int32_t)(colptr1[j] - 1);
+        if (j > 0 && colptr1[j] < colptr1[j - 1]) return fail(QOB_EARG, "colptr not monotone");
+    }
+    S->csc.idx.resize(nnz);
+    S->csc.val.resize(nnz);
+    for (int64_t p = 0; p < nnz; ++p) {
+        const int64_t r = rowval1[p] - 1;
+        if (r < 0 || r >= m) return fail(QOB_EARG, "rowval[%lld] = %lld out of range", (long long)p, (long long)rowval1[p]);
+        S->csc.idx[p] = (int32_t)r;
+        S->csc.val[p] = cdv(nzval[p]);
+    }
+    S->csr = transpose_rows(S->csc, false);
+    auto node = std::make_shared<OpNode>();
+    node->kind = K_SPARSE;
+    node->m = m;
+    node->n = n;
+    node->sp = S;
+    *out = wrap(node);
+    return QOB_OK;
+}
+
+qob_status qob_dense_create(qob_ctx ctx, int64_t m, int64_t n, const qob_c64 *data, qob_op *out) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    if (m < 0 || n < 0) return fail(QOB_EARG, "bad dims");
+    if (m * n > 0 && !data) return fail(QOB_EARG, "null data");
+    auto D = std::make_shared<DenseData>();
+    D->m = m;
+    D->n = n;
+    D->a.resize((size_t)(m * n));
+    for (int64_t i = 0; i < m * n; ++i) D->a[i] = cdv(data[i]);
+    auto node = std::make_shared<OpNode>();
+    node->kind = K_DENSE;
+    node->m = m;
+    node->n = n;
+    node->de = D;
+    *out = wrap(node);
+    return QOB_OK;
+}
+
+qob_status qob_eye_create(qob_ctx ctx, int64_t m, int64_t n, qob_op *out) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    if (m < 0 || n < 0) return fail(QOB_EARG, "bad dims");
+    auto node = std::make_shared<OpNode>();
+    node->kind = K_EYE;
+    node->m = m;
+    node->n = n;
+    *out = wrap(node);
+    return QOB_OK;
+}
+
+static std::shared_ptr<OpNode> adjoint_node(const std::shared_ptr<OpNode> &a) {
+    auto r = std::make_shared<OpNode>();
+    r->kind = a->kind;
+    r->m = a->n;
+    r->n = a->m;
+    switch (a->kind) {
+    case K_SPARSE: r->sp = a->sp; r->var = a->var ^ V_H; break;
+    case K_DENSE: r->de = a->de; r->var = a->var ^ V_H; break;
+    case K_EYE: break;
+    case K_LT:
+        r->shl = a->shr;
+        r->shr = a->shl;
+        r->factor = std::conj(a->factor);
+        r->fsite = a->fsite;
+        for (auto &f : a->fop) r->fop.push_back(adjoint_node(f));
+        break;
+    case K_LS:
+        for (auto &c : a->coef) r->coef.push_back(std::conj(c));
+        for (auto &t : a->terms) r->terms.push_back(adjoint_node(t));
+        break;
+    }
+    return r;
+}
+
+qob_status qob_op_adjoint(qob_ctx ctx, qob_op a, qob_op *out) {
+    if (!ctx || !a || !out) return fail(QOB_EARG, "null argument");
+    *out = wrap(adjoint_node(a->node));
+    return QOB_OK;
+}
+
+qob_status qob_lazytensor_create(qob_ctx ctx, int nsites, const int64_t *shape_l, const int64_t *shape_r,
+                                 int nfactors, const int64_t *indices1, const qob_op *factors, qob_c64 factor,
+                                 qob_op *out) {
+    if (!ctx || !out || nsites < 1 || !shape_l || !shape_r) return fail(QOB_EARG, "bad arguments");
+    if (nfactors < 0 || (nfactors > 0 && (!indices1 || !factors))) return fail(QOB_EARG, "bad factor arguments");
+    auto node = std::make_shared<OpNode>();
+    node->kind = K_LT;
+    node->shl.assign(shape_l, shape_l + nsites);
+    node->shr.assign(shape_r, shape_r + nsites);
+    node->m = prod(node->shl, 0, nsites);
+    node->n = prod(node->shr, 0, nsites);
+    for (int s = 0; s < nsites; ++s)
+        if (shape_l[s] < 1 || shape_r[s] < 1) return fail(QOB_EARG, "site dims must be positive");
+    node->factor = cdv(factor);
+    for (int i = 0; i < nfactors; ++i) {
+        const int64_t s = indices1[i] - 1;
+        if (s < 0 || s >= nsites) return fail(QOB_EARG, "index %lld out of range 1..%d", (long long)indices1[i], nsites);
+        if (i > 0 && indices1[i] <= indices1[i - 1])
+            return fail(QOB_EARG, "indices must be sorted and unique");  // src/operators_lazytensor.jl:26
+        if (!factors[i]) return fail(QOB_EARG, "null factor");
+        const OpNode &f = *factors[i]->node;
+        if (f.kind == K_LT || f.kind == K_LS) return fail(QOB_EUNSUPPORTED, "LazyTensor factors must be data operators");
+        if (f.m != shape_l[s] || f.n != shape_r[s])
+            return fail(QOB_EBASIS, "factor %d is %lldx%lld but site %lld is %lldx%lld", i + 1, (long long)f.m,
+                        (long long)f.n, (long long)(s + 1), (long long)shape_l[s], (long long)shape_r[s]);
+        node->fsite.push_back((int)s);
+        node->fop.push_back(factors[i]->node);
+    }
+    *out = wrap(node);
+    return QOB_OK;
+}
+
+qob_status qob_lazysum_create(qob_ctx ctx, int64_t m, int64_t n, int nterms, const qob_c64 *factors,
+                              const qob_op *operators, qob_op *out) {
+    if (!ctx || !out || nterms < 0) return fail(QOB_EARG, "bad arguments");
+    if (nterms > 0 && (!factors || !operators)) return fail(QOB_EARG, "null terms");
+    auto node = std::make_shared<OpNode>();
+    node->kind = K_LS;
+    node->m = m;
+    node->n = n;
+    const OpNode *first_lt = nullptr;
+    for (int t = 0; t < nterms; ++t) {
+        if (!operators[t]) return fail(QOB_EARG, "null term");
+        auto &x = operators[t]->node;
+        if (x->m != m || x->n != n) return fail(QOB_EBASIS, "term %d is %lldx%lld, sum is %lldx%lld", t + 1,
+                                                (long long)x->m, (long long)x->n, (long long)m, (long long)n);
+        if (x->kind == K_LT) {
+            if (first_lt && (first_lt->shl != x->shl || first_lt->shr != x->shr))
+                return fail(QOB_EBASIS, "LazySum terms on different composite bases");
+            first_lt = x.get();
+        }
+        node->coef.push_back(cdv(factors[t]));
+        node->terms.push_back(x);
+    }
+    *out = wrap(node);
+    return QOB_OK;
+}
+
+qob_status qob_lazysum_set_factors(qob_ctx ctx, qob_op sum, const qob_c64 *factors) {
+    if (!ctx || !sum || sum->node->kind != K_LS) return fail(QOB_EARG, "not a LazySum");
+    OpNode &n = *sum->node;
+    if (!n.terms.empty() && !factors) return fail(QOB_EARG, "null factors");
+    // plans of the old coefficients may still be read by queued work
+    QOB_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> g(n.mu);
+    for (size_t t = 0; t < n.terms.size(); ++t) n.coef[t] = cdv(factors[t]);
+    n.version++;
+    n.gplans.clear();
+    n.sums.clear();
+    return QOB_OK;
+}
+
+qob_status qob_op_shape(qob_op a, int64_t *m, int64_t *n) {
+    if (!a) return fail(QOB_EARG, "null operator");
+    if (m) *m = a->node->m;
+    if (n) *n = a->node->n;
+    return QOB_OK;
+}
+
+qob_status qob_op_release(qob_op a) {
+    delete a;
+    return QOB_OK;
+}
+
+// ------------------------------------------------------------------ mul!
+static qob_status check_common(qob_ctx ctx, qob_dtype dt, qob_op a, const void *b, void *r) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    if (!a) return fail(QOB_EARG, "null operator");
+    if (dt != QOB_C64 && dt != QOB_C32) return fail(QOB_EARG, "bad dtype %d", (int)dt);
+    if (b == r && b) return fail(QOB_EALIAS, "output must not be aliased with input");
+    return QOB_OK;
+}
+
+qob_status qob_mul_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *b, int64_t nb, void *result, int64_t nr,
+                       qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    QOB_TRY(check_common(ctx, dt, a, b, result));
+    OpNode &op = *a->node;
+    if (op.n != nb || op.m != nr)
+        return fail(QOB_EDIM, "Output dimensions do not match A*B: op %lldx%lld, b %lld, result %lld",
+                    (long long)op.m, (long long)op.n, (long long)nb, (long long)nr);
+    Apply ap{ctx, strm(ctx, s), dt, 1, 1, false};
+    return apply_op(ap, op, b, result, cdv(alpha), cdv(beta));
+}
+
+qob_status qob_mul_bra(qob_ctx ctx, qob_dtype dt, const void *a, int64_t na, qob_op b, void *result, int64_t nr,
+                       qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    QOB_TRY(check_common(ctx, dt, b, a, result));
+    OpNode &op = *b->node;
+    if (op.m != na || op.n != nr)
+        return fail(QOB_EDIM, "Output dimensions do not match a*B: a %lld, op %lldx%lld, result %lld", (long long)na,
+                    (long long)op.m, (long long)op.n, (long long)nr);
+    Apply ap{ctx, strm(ctx, s), dt, 1, 1, true};
+    return apply_op(ap, op, a, result, cdv(alpha), cdv(beta));
+}
+
+qob_status qob_mul_op_left(qob_ctx ctx, qob_dtype dt, qob_op a, const void *B, int64_t rows_b, int64_t cols, void *R,
+                           int64_t rows_r, qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    QOB_TRY(check_common(ctx, dt, a, B, R));
+    OpNode &op = *a->node;
+    if (op.n != rows_b || op.m != rows_r || cols < 0)
+        return fail(QOB_EDIM, "Output dimensions do not match A*B: op %lldx%lld, B %lldx%lld, R %lldx%lld",
+                    (long long)op.m, (long long)op.n, (long long)rows_b, (long long)cols, (long long)rows_r,
+                    (long long)cols);
+    if (cols == 0) return QOB_OK;
+    Apply ap{ctx, strm(ctx, s), dt, 1, cols, false};
+    return apply_op(ap, op, B, R, cdv(alpha), cdv(beta));
+}
+
+qob_status qob_mul_op_right(qob_ctx ctx, qob_dtype dt, const void *B, int64_t rows, int64_t cols_b, qob_op a, void *R,
+                            int64_t cols_r, qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    QOB_TRY(check_common(ctx, dt, a, B, R));
+    OpNode &op = *a->node;
+    if (op.m != cols_b || op.n != cols_r || rows < 0)
+        return fail(QOB_EDIM, "Output dimensions do not match B*A: B %lldx%lld, op %lldx%lld, R %lldx%lld",
+                    (long long)rows, (long long)cols_b, (long long)op.m, (long long)op.n, (long long)rows,
+                    (long long)cols_r);
+    if (rows == 0) return QOB_OK;
+    Apply ap{ctx, strm(ctx, s), dt, rows, 1, true};
+    if (rows == 1) ap.lead = 1;  // a single row is a Bra: contiguous gather axis
+    return apply_op(ap, op, B, R, cdv(alpha), cdv(beta));
+}
+
+qob_status qob_gemm(qob_ctx ctx, qob_dtype dt, int transa, int64_t m, int64_t n, int64_t k, qob_c64 alpha,
+                    const void *A, const void *B, qob_c64 beta, void *C, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    if (C == A || C == B) return fail(QOB_EALIAS, "output must not be aliased with input");
+    if (m < 0 || n < 0 || k < 0) return fail(QOB_EDIM, "negative dimension");
+    const int64_t lda = transa ? k : m;
+    return blas_gemm(ctx, strm(ctx, s), dt, transa ? 'T' : 'N', 'N', m, n, k, cdv(alpha), A, lda, 0, B, k, 0,
+                     cdv(beta), C, m, 0, 1);
+}
+
+// ----------------------------------------------------- elementwise / reduce
+qob_status qob_scale(qob_ctx ctx, qob_dtype dt, int64_t n, qob_c64 a, void *x, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    hipError_t e = launch_scale(dt, n, d2(a), x, strm(ctx, s));
+    if (e != hipSuccess) return fail(QOB_EHIP, "scale: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_axpy(qob_ctx ctx, qob_dtype dt, int64_t n, qob_c64 a, const void *x, void *y, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    if (x == y) return fail(QOB_EALIAS, "axpy with x == y");
+    hipError_t e = launch_axpy(dt, n, d2(a), x, y, strm(ctx, s));
+    if (e != hipSuccess) return fail(QOB_EHIP, "axpy: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_convert(qob_ctx ctx, qob_dtype from, const void *src, qob_dtype to, void *dst, int64_t n, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    hipError_t e = launch_convert(from, src, to, dst, n, strm(ctx, s));
+    if (e != hipSuccess) return fail(QOB_EHIP, "convert: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+static qob_status red_ws(qob_ctx ctx, hipStream_t st, int64_t n, double2 *&part) {
+    void *p;
+    QOB_TRY(ws_get(ctx, st, WS_RED, (size_t)reduce_blocks(n) * sizeof(double2), p));
+    part = (double2 *)p;
+    return QOB_OK;
+}
+
+qob_status qob_dot_d(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x, const void *y, void *dout, qob_stream s) {
+    if (!ctx || !dout) return fail(QOB_EARG, "null argument");
+    hipStream_t st = strm(ctx, s);
+    double2 *part;
+    QOB_TRY(red_ws(ctx, st, n, part));
+    hipError_t e = launch_dot(dt, n, x, y, part, (double2 *)dout, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "dot: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_norm_d(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x, void *dout, qob_stream s) {
+    if (!ctx || !dout) return fail(QOB_EARG, "null argument");
+    hipStream_t st = strm(ctx, s);
+    double2 *part;
+    QOB_TRY(red_ws(ctx, st, n, part));
+    hipError_t e = launch_nrm2(dt, n, x, part, (double *)dout, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "norm: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_trace_d(qob_ctx ctx, qob_dtype dt, int64_t n, const void *A, void *dout, qob_stream s) {
+    if (!ctx || !dout) return fail(QOB_EARG, "null argument");
+    hipStream_t st = strm(ctx, s);
+    double2 *part;
+    QOB_TRY(red_ws(ctx, st, n, part));
+    hipError_t e = launch_trace(dt, n, A, part, (double2 *)dout, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "trace: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+static qob_status scal_ws(qob_ctx ctx, hipStream_t st, void *&p) { return ws_get(ctx, st, WS_SCAL, 64, p); }
+
+}  // extern "C"
+
+template <typename F>
+static qob_status reduce_to_host(qob_ctx ctx, qob_stream s, void *host, size_t bytes, F f) {
+    hipStream_t st = strm(ctx, s);
+    void *d;
+    QOB_TRY(scal_ws(ctx, st, d));
+    QOB_TRY(f(d));
+    QOB_HIP(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, st));
+    QOB_HIP(hipStreamSynchronize(st));
+    return QOB_OK;
+}
+
+extern "C" {
+
+qob_status qob_dot(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x, const void *y, qob_c64 *out, qob_stream s) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    return reduce_to_host(ctx, s, out, 16, [&](void *d) { return qob_dot_d(ctx, dt, n, x, y, d, s); });
+}
+
+qob_status qob_norm(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x, double *out, qob_stream s) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    return reduce_to_host(ctx, s, out, 8, [&](void *d) { return qob_norm_d(ctx, dt, n, x, d, s); });
+}
+
+qob_status qob_trace(qob_ctx ctx, qob_dtype dt, int64_t n, const void *A, qob_c64 *out, qob_stream s) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    return reduce_to_host(ctx, s, out, 16, [&](void *d) { return qob_trace_d(ctx, dt, n, A, d, s); });
+}
+
+qob_status qob_normalize_ket(qob_ctx ctx, qob_dtype dt, int64_t n, void *x, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    hipStream_t st = strm(ctx, s);
+    void *d;
+    QOB_TRY(scal_ws(ctx, st, d));
+    QOB_TRY(qob_norm_d(ctx, dt, n, x, d, s));
+    hipError_t e = launch_scale_by_inv_norm(dt, n, (const double *)d, x, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "normalize: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_normalize_op(qob_ctx ctx, qob_dtype dt, int64_t n, void *A, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    hipStream_t st = strm(ctx, s);
+    void *d;
+    QOB_TRY(scal_ws(ctx, st, d));
+    QOB_TRY(qob_trace_d(ctx, dt, n, A, d, s));
+    hipError_t e = launch_scale_by_inv_trace(dt, n, n, (const double2 *)d, A, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "normalize: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_expect_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *psi, int64_t n, qob_c64 *out, qob_stream s) {
+    if (!ctx || !a || !out) return fail(QOB_EARG, "null argument");
+    OpNode &op = *a->node;
+    if (op.m != n || op.n != n) return fail(QOB_EDIM, "expect: operator %lldx%lld on state %lld", (long long)op.m, (long long)op.n, (long long)n);
+    hipStream_t st = strm(ctx, s);
+    // tmp = op * psi; <psi|tmp>  (src/operators.jl:119)
+    void *tmp;
+    QOB_TRY(ws_get(ctx, st, WS_EXP, (size_t)n * esize(dt), tmp));
+    Apply ap{ctx, st, dt, 1, 1, false};
+    QOB_TRY(apply_op(ap, op, psi, tmp, cd(1, 0), cd(0, 0)));
+    return qob_dot(ctx, dt, n, psi, tmp, out, s);
+}
+
+qob_status qob_expect_op(qob_ctx ctx, qob_dtype dt, qob_op a, const void *rho, int64_t n, qob_c64 *out, qob_stream s) {
+    if (!ctx || !a || !out) return fail(QOB_EARG, "null argument");
+    OpNode &op = *a->node;
+    if (op.m != n || op.n != n) return fail(QOB_EDIM, "expect: operator %lldx%lld on state %lldx%lld", (long long)op.m, (long long)op.n, (long long)n, (long long)n);
+    hipStream_t st = strm(ctx, s);
+    // tr(op * rho) (src/operators_dense.jl:223-230)
+    void *tmp;
+    QOB_TRY(ws_get(ctx, st, WS_EXP, (size_t)n * n * esize(dt), tmp));
+    Apply ap{ctx, st, dt, 1, n, false};
+    QOB_TRY(apply_op(ap, op, rho, tmp, cd(1, 0), cd(0, 0)));
+    return qob_trace(ctx, dt, n, tmp, out, s);
+}
+
+}  // extern "C"

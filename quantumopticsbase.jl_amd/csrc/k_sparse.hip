@@ -1,0 +1,202 @@
+// Sparse x dense kernels: the SparseOperator branch of mul!.
+//
+// Left  R = alpha * X * B + beta * R   (src/sparsematrix.jl:25-47, 99-113;
+//        gemv src/sparsematrix.jl:190-214; adjoint src/sparsematrix.jl:73-96,148-163)
+// Right R = alpha * B * X + beta * R   (src/sparsematrix.jl:115-146;
+//        vector x CSC src/sparsematrix.jl:216-238; adjoint src/sparsematrix.jl:49-71,165-175)
+//
+// The reference scatters (walks CSC columns and adds into result rows).  On
+// the GPU both directions are written as gathers so that every output
+// element is owned by one lane and written exactly once, fused with the
+// beta scaling (the reference's separate fill!/rmul! pass):
+//   left : lanes own rows of R and walk the rows of X, held as sliced ELL
+//          (64-row slices, one wavefront per slice, coalesced index loads);
+//   right: lanes own rows of R (contiguous), one output column per step,
+//          the column's CSC entries are wave-uniform (scalar loads) and each
+//          gathers a contiguous run of a column of B.
+// The adjoint variants use the same kernels on the conjugate-transposed
+// tables built at upload time.
+#include "qob_internal.h"
+
+namespace qob {
+
+namespace {
+
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id b runs on
+// XCD b % 8).  xcd_remap turns b into a logical id such that each XCD owns a
+// contiguous range of logical ids, so work that shares data shares an L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
+    const uint32_t x = b & 7u, i = b >> 3;
+    const uint32_t q = G >> 3, r = G & 7u;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+template <typename T> __device__ __forceinline__ T to_t(double2 a) {
+    T r; r.x = (typename real_of<T>::type)a.x; r.y = (typename real_of<T>::type)a.y; return r;
+}
+
+// ---- left: sliced ELL row gather.  Block = 256 lanes = 4 slices of 64 rows.
+// Logical block = (column group cg, slice group sg), slice group fastest so
+// one XCD walks all row slices of the same columns of B.
+template <typename T, int CJ>
+__global__ void __launch_bounds__(256) k_sell_left(int nslices, int64_t m, const int32_t *__restrict__ slice_off,
+                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                   const T *__restrict__ B, int64_t ldb, int64_t cols,
+                                                   T *__restrict__ R, int64_t ldr, double2 alpha_, double2 beta_,
+                                                   int beta_zero, uint32_t nsg) {
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = xcd_remap(blockIdx.x, G);
+    const uint32_t sg = lb % nsg, cg = lb / nsg;
+    const int slice = (int)(sg * 4 + (threadIdx.x >> 6));
+    const int r = threadIdx.x & 63;
+    if (slice >= nslices) return;
+    const int64_t row = (int64_t)slice * 64 + r;
+    const int64_t j0 = (int64_t)cg * CJ;
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+
+    T acc[CJ];
+#pragma unroll
+    for (int jj = 0; jj < CJ; ++jj) acc[jj] = czero<T>();
+
+    const int32_t o0 = slice_off[slice];
+    const int w = (slice_off[slice + 1] - o0) >> 6;
+    for (int p = 0; p < w; ++p) {
+        const int32_t c = col[o0 + p * 64 + r];
+        if (c >= 0) {
+            const T v = val[o0 + p * 64 + r];
+#pragma unroll
+            for (int jj = 0; jj < CJ; ++jj) {
+                if (j0 + jj < cols) acc[jj] = cfma(v, B[(int64_t)c + (j0 + jj) * ldb], acc[jj]);
+            }
+        }
+    }
+    if (row < m) {
+#pragma unroll
+        for (int jj = 0; jj < CJ; ++jj) {
+            if (j0 + jj < cols) {
+                T *o = R + row + (j0 + jj) * ldr;
+                *o = axpby(alpha, acc[jj], beta, o, beta_zero != 0);
+            }
+        }
+    }
+}
+
+// ---- right: CSC column gather, lanes along rows of B/R.
+// Logical block = (row block rb, column group cg), column group fastest.
+template <typename T, int RB>
+__global__ void __launch_bounds__(RB) k_csc_right(int64_t rows, int64_t n, const int32_t *__restrict__ colptr,
+                                                  const int32_t *__restrict__ rowind, const T *__restrict__ val,
+                                                  const T *__restrict__ B, T *__restrict__ R, double2 alpha_,
+                                                  double2 beta_, int beta_zero, uint32_t ncg, int cc, int remap) {
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
+    const uint32_t cg = lb % ncg, rb = lb / ncg;
+    const int64_t i = (int64_t)rb * RB + threadIdx.x;
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+    const bool ok = i < rows;
+    const int64_t c0 = (int64_t)cg * cc;
+    const int64_t c1 = (c0 + cc < n) ? c0 + cc : n;
+    for (int64_t c = c0; c < c1; ++c) {
+        const int32_t p0 = colptr[c], p1 = colptr[c + 1];
+        T acc = czero<T>();
+        if (ok) {
+            for (int32_t p = p0; p < p1; ++p) {
+                acc = cfma(val[p], B[i + (int64_t)rowind[p] * rows], acc);
+            }
+            T *o = R + i + c * rows;
+            *o = axpby(alpha, acc, beta, o, beta_zero != 0);
+        }
+    }
+}
+
+// ---- right with few rows (Bra, thin B): lanes along output columns.
+template <typename T>
+__global__ void __launch_bounds__(256) k_csc_right_thin(int64_t rows, int64_t n, const int32_t *__restrict__ colptr,
+                                                        const int32_t *__restrict__ rowind, const T *__restrict__ val,
+                                                        const T *__restrict__ B, T *__restrict__ R, double2 alpha_,
+                                                        double2 beta_, int beta_zero) {
+    const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (c >= n) return;
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+    const int32_t p0 = colptr[c], p1 = colptr[c + 1];
+    for (int64_t i = 0; i < rows; ++i) {
+        T acc = czero<T>();
+        for (int32_t p = p0; p < p1; ++p) acc = cfma(val[p], B[i + (int64_t)rowind[p] * rows], acc);
+        T *o = R + i + c * rows;
+        *o = axpby(alpha, acc, beta, o, beta_zero != 0);
+    }
+}
+
+}  // namespace
+
+// Tunables (set once from the host; defaults chosen from measurements).
+int g_sell_cj = 4;
+int g_csc_rb = 256;
+int g_csc_cc = 8;
+int g_csc_remap = 0;
+
+hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols, void *R,
+                            double2 alpha, double2 beta, hipStream_t s) {
+    if (X.m == 0 || cols == 0) return hipSuccess;
+    const int bz = (beta.x == 0.0 && beta.y == 0.0);
+    const uint32_t nsg = (uint32_t)((X.nslices + 3) / 4);
+    const int cj = (cols == 1) ? 1 : g_sell_cj;
+    const uint32_t ncg = (uint32_t)((cols + cj - 1) / cj);
+    const uint64_t G = (uint64_t)nsg * ncg;
+    if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
+#define QOB_SELL(TT, CJV)                                                                                 \
+    hipLaunchKernelGGL((k_sell_left<TT, CJV>), dim3((uint32_t)G), dim3(256), 0, s, X.nslices, X.m,       \
+                       X.slice_off, X.col, (const TT *)X.val, (const TT *)B, X.n, cols, (TT *)R, X.m, alpha, \
+                       beta, bz, nsg)
+    if (dt == QOB_C64) {
+        if (cj == 1) QOB_SELL(double2, 1);
+        else if (cj == 2) QOB_SELL(double2, 2);
+        else if (cj == 4) QOB_SELL(double2, 4);
+        else QOB_SELL(double2, 8);
+    } else {
+        if (cj == 1) QOB_SELL(float2, 1);
+        else if (cj == 2) QOB_SELL(float2, 2);
+        else if (cj == 4) QOB_SELL(float2, 4);
+        else QOB_SELL(float2, 8);
+    }
+#undef QOB_SELL
+    return hipGetLastError();
+}
+
+hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_t rows, void *R,
+                            double2 alpha, double2 beta, hipStream_t s) {
+    if (X.n == 0 || rows == 0) return hipSuccess;
+    const int bz = (beta.x == 0.0 && beta.y == 0.0);
+    if (rows < 32) {
+        const uint32_t g = (uint32_t)((X.n + 255) / 256);
+        if (dt == QOB_C64)
+            hipLaunchKernelGGL(k_csc_right_thin<double2>, dim3(g), dim3(256), 0, s, rows, X.n, X.colptr, X.row,
+                               (const double2 *)X.val, (const double2 *)B, (double2 *)R, alpha, beta, bz);
+        else
+            hipLaunchKernelGGL(k_csc_right_thin<float2>, dim3(g), dim3(256), 0, s, rows, X.n, X.colptr, X.row,
+                               (const float2 *)X.val, (const float2 *)B, (float2 *)R, alpha, beta, bz);
+        return hipGetLastError();
+    }
+    const int rb = g_csc_rb;
+    const int cc = g_csc_cc;
+    const uint32_t nrb = (uint32_t)((rows + rb - 1) / rb);
+    const uint32_t ncg = (uint32_t)((X.n + cc - 1) / cc);
+    const uint64_t G = (uint64_t)nrb * ncg;
+    if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
+#define QOB_CSC(TT, RBV)                                                                                  \
+    hipLaunchKernelGGL((k_csc_right<TT, RBV>), dim3((uint32_t)G), dim3(RBV), 0, s, rows, X.n, X.colptr, X.row, \
+                       (const TT *)X.val, (const TT *)B, (TT *)R, alpha, beta, bz, ncg, cc, g_csc_remap)
+    if (dt == QOB_C64) {
+        if (rb == 64) QOB_CSC(double2, 64);
+        else if (rb == 128) QOB_CSC(double2, 128);
+        else QOB_CSC(double2, 256);
+    } else {
+        if (rb == 64) QOB_CSC(float2, 64);
+        else if (rb == 128) QOB_CSC(float2, 128);
+        else QOB_CSC(float2, 256);
+    }
+#undef QOB_CSC
+    return hipGetLastError();
+}
+
+}  // namespace qob

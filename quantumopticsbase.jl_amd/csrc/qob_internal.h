@@ -1,0 +1,178 @@
+// Internal declarations shared by the HIP kernels and the host API of
+// libqob_hip.so.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+#include "../../include/qob_hip.h"
+
+namespace qob {
+
+// ------------------------------------------------------------ complex math
+// Interleaved (re, im) pairs, the layout of Julia's ComplexF64/ComplexF32.
+template <typename R> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+template <typename T> struct real_of;
+template <> struct real_of<double2> { using type = double; };
+template <> struct real_of<float2> { using type = float; };
+
+__host__ __device__ inline double2 mk(double re, double im) { return make_double2(re, im); }
+__host__ __device__ inline float2 mk(float re, float im) { return make_float2(re, im); }
+
+template <typename T> __host__ __device__ inline T czero() { T z; z.x = 0; z.y = 0; return z; }
+
+// a * b
+template <typename T> __device__ __forceinline__ T cmul(T a, T b) {
+    T r;
+    r.x = fma(a.x, b.x, -a.y * b.y);
+    r.y = fma(a.x, b.y, a.y * b.x);
+    return r;
+}
+// c + a * b
+template <typename T> __device__ __forceinline__ T cfma(T a, T b, T c) {
+    T r;
+    r.x = fma(a.x, b.x, fma(-a.y, b.y, c.x));
+    r.y = fma(a.x, b.y, fma(a.y, b.x, c.y));
+    return r;
+}
+template <typename T> __device__ __forceinline__ T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T> __device__ __forceinline__ T cconj(T a) { a.y = -a.y; return a; }
+
+// alpha * acc + beta * old, with beta == 0 discarding old (NaN-safe).
+template <typename T> __device__ __forceinline__ T axpby(T alpha, T acc, T beta, const T *old, bool beta_zero) {
+    T r = cmul(alpha, acc);
+    if (!beta_zero) r = cfma(beta, *old, r);
+    return r;
+}
+
+// ------------------------------------------------------ fast integer divide
+// q = n / d for 0 <= n < 2^31 with a host-computed multiplier:
+//   l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+//   q = (umulhi(n, m) + n) >> l.
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, section 4.)  d == 2^l gives m == 1 and an
+// exact shift.  Verified exhaustively against '/' on the host for the
+// divisors used by the tests.
+struct FastDiv {
+    uint32_t d, m, l;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.l = l;
+    f.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+    return f;
+}
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m, uint32_t l) {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t t = __umulhi(n, m);
+#else
+    uint32_t t = (uint32_t)(((uint64_t)n * m) >> 32);
+#endif
+    return (t + n) >> l;
+}
+
+// ---------------------------------------------------------- lazy gather plan
+// A compiled sum of pure-sparse tensor-product terms (LazySum of
+// LazyTensors whose factors are all sparse or identities), in gather form:
+//   out[J] = alpha * sum_t sum_{entries of row jloc_t(J)} v * in[K] + beta * out[J]
+// The term program is a flat int32 array walked with wave-uniform indices.
+//
+// Program layout (per term group):
+//   [0] kind    : 0 = expanded local CSR, 1 = product of per-site tables
+//   [1] flags   : bit0 = diagonal (all entries have K == J)
+//   [2] q       : number of factor sites
+//   [3] rowptr  : offset of the local row pointers in the rowptr pool (kind 0)
+//                 or of the per-site tables in the entry pool (kind 1)
+//   [4] fmask   : bitmask of factor sites (non-square shapes only)
+//   then q site records of GP_SITE ints:
+//     stride_l, m_stride, l_stride, d_l, m_d, l_d, lstride, tbl_off, stride_r
+//   Digit of site s: j = (J / stride_l) mod d_l (l = output space,
+//   r = input space; they differ only for non-square shapes).
+//   kind 0: jloc = sum lstride * j; entries rowptr[jloc] .. rowptr[jloc+1]
+//   kind 1: site table entry tbl_off + j holds (koff, v) with v == 0 meaning
+//           an empty row.
+enum { GP_HDR = 5, GP_SITE = 9 };
+
+struct GatherPlanDev {
+    const int32_t *prog = nullptr;
+    int nterms = 0;
+    const int32_t *rowptr = nullptr;   // local CSR row pointers
+    const int32_t *koff = nullptr;     // per entry: delta (square) or r-offset
+    const void *vals = nullptr;        // per entry value (dtype of the plan)
+    // non-square support: all site digits are needed
+    int nsites = 0;
+    const int32_t *site_info = nullptr;  // per site: stride_l, m, l, d_l, m_d, l_d, stride_r, d_r
+    bool square = true;
+};
+
+// ------------------------------------------------------------ launchers
+// All launchers return hipSuccess or the launch error; they never allocate.
+
+// elementwise (k_blas1.hip)
+hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t s);
+hipError_t launch_axpy(qob_dtype dt, int64_t n, double2 a, const void *x, void *y, hipStream_t s);
+hipError_t launch_axpby_copy(qob_dtype dt, int64_t n, double2 a, const void *x, double2 b, void *y, hipStream_t s);
+hipError_t launch_embed(qob_dtype dt, const void *in, int64_t ri, int64_t ci, void *out, int64_t ro, int64_t co,
+                        double2 a, double2 b, hipStream_t s);
+hipError_t launch_convert(qob_dtype from, const void *src, qob_dtype to, void *dst, int64_t n, hipStream_t s);
+// reductions into a device double2/double; partials must hold reduce_partials() doubles*2
+int reduce_blocks(int64_t n);
+hipError_t launch_dot(qob_dtype dt, int64_t n, const void *x, const void *y, double2 *partials, double2 *out, hipStream_t s);
+hipError_t launch_nrm2(qob_dtype dt, int64_t n, const void *x, double2 *partials, double *out, hipStream_t s);
+hipError_t launch_trace(qob_dtype dt, int64_t n, const void *A, double2 *partials, double2 *out, hipStream_t s);
+// x *= 1/|norm| or 1/trace taken from a device scalar
+hipError_t launch_scale_by_inv_norm(qob_dtype dt, int64_t n, const double *norm, void *x, hipStream_t s);
+hipError_t launch_scale_by_inv_trace(qob_dtype dt, int64_t n, int64_t ld, const double2 *tr, void *A, hipStream_t s);
+
+// sparse (k_sparse.hip)
+// Sliced-ELL gather table of the rows of an m x n sparse matrix X:
+// slices of 64 rows, slice k has width w_k; entry (p, r) of slice k sits at
+// slice_off[k] + p*64 + r.  Column index -1 marks padding.
+struct SellDev {
+    int64_t m = 0, n = 0;
+    int nslices = 0;
+    const int32_t *slice_off = nullptr;  // nslices+1 (in entries)
+    const int32_t *col = nullptr;
+    const void *val = nullptr;            // CF64 or CF32
+};
+// CSC of X (columns of X as lists of (row, value)).
+struct CscDev {
+    int64_t m = 0, n = 0;
+    const int32_t *colptr = nullptr, *row = nullptr;
+    const void *val = nullptr;
+};
+// R (m x cols) = alpha * X * B + beta * R, B (n x cols)
+hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols,
+                            void *R, double2 alpha, double2 beta, hipStream_t s);
+// R (rows x n) = alpha * B * X + beta * R, B (rows x m)
+hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_t rows,
+                            void *R, double2 alpha, double2 beta, hipStream_t s);
+
+// lazy gather (k_lazy.hip)
+// out (n x nbatch, J fastest) = alpha * G in + beta * out, J = row index
+hipError_t launch_gather_rows(qob_dtype dt, const GatherPlanDev &P, const void *in, int64_t n_in,
+                              void *out, int64_t n_out, int64_t nbatch, double2 alpha,
+                              double2 beta, hipStream_t s);
+// out (nrows x n_out) = alpha * (G applied along the column index) + beta*out:
+// out[:, J] = sum entries v * in[:, K]
+hipError_t launch_gather_cols(qob_dtype dt, const GatherPlanDev &P, const void *in, int64_t n_in,
+                              void *out, int64_t n_out, int64_t nrows, double2 alpha,
+                              double2 beta, hipStream_t s);
+
+// dense mode-k (k_modek.hip)
+// out viewed (n1, d_out, n3), in viewed (n1, d_in, n3);
+// out = alpha * A x_2 in + beta * out, A is d_out x d_in column-major.
+hipError_t launch_modek(qob_dtype dt, const void *A, int64_t d_out, int64_t d_in, const void *in,
+                        void *out, int64_t n1, int64_t n3, double2 alpha, double2 beta,
+                        hipStream_t s);
+bool modek_supported(int64_t d_out, int64_t d_in);
+
+}  // namespace qob

@@ -1,0 +1,285 @@
+"""GPU parity: every mul! path through libqob_hip.so against the oracle.
+
+Bar: relative Frobenius error <= 1e-12 for ComplexF64, <= 1e-5 for
+ComplexF32, on the same seeded inputs.  Cases follow the reference tests
+(test/test_operators_sparse.jl:318-473, test_operators_lazytensor.jl:237-496,
+test_operators_lazysum.jl:255-347, test_operators_dense.jl:342-419).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import qob_oracle as O
+import qobhip as Q
+from test_oracle import AB, LT_CASES, cg, rand_lt, rs
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def rel(a, b):
+    return O.rel_frobenius(a, b)
+
+
+def gb(d):
+    return Q.GenericBasis(d)
+
+
+def comp(shape):
+    return Q.tensor(*[gb(d) for d in shape]) if len(shape) > 1 else gb(shape[0])
+
+
+def to_q(op, bl=None, br=None):
+    """Build the qobhip operator equal to an oracle operator."""
+    if isinstance(op, O.OSparse):
+        M = sp.csc_matrix((op.nzval, op.rowval - 1, op.colptr - 1), shape=(op.m, op.n))
+        if op.adj:
+            return Q.dagger(Q.SparseOperator(br or gb(op.n), bl or gb(op.m), M))
+        return Q.SparseOperator(bl or gb(op.m), br or gb(op.n), M)
+    if isinstance(op, O.ODense):
+        return Q.DenseOperator(bl or gb(op.shape[0]), br or gb(op.shape[1]), op.mat())
+    if isinstance(op, O.OEye):
+        return Q.identityoperator(bl or gb(op.m), br or gb(op.n))
+    if isinstance(op, O.OLazyTensor):
+        bl2, br2 = comp(op.shape_l), comp(op.shape_r)
+        ops = tuple(to_q(o, gb(op.shape_l[i - 1]), gb(op.shape_r[i - 1])) for o, i in zip(op.operators, op.indices))
+        return Q.LazyTensor(bl2, br2, op.indices, ops, op.factor)
+    if isinstance(op, O.OLazySum):
+        bl2 = bl or gb(op.m)
+        br2 = br or gb(op.n)
+        for o in op.operators:
+            if isinstance(o, O.OLazyTensor):
+                bl2, br2 = comp(o.shape_l), comp(o.shape_r)
+        return Q.LazySum(bl2, br2, op.factors, tuple(to_q(o, bl2, br2) for o in op.operators))
+    raise TypeError(type(op))
+
+
+def run_all_paths(oop, qop, g, alpha, beta, dtype=np.complex128, tol=TOL64, ncols=4):
+    A_m, A_n = oop.shape
+    bl, br = qop.basis_l, qop.basis_r
+    # Ket
+    b = cg(g, A_n)
+    r0 = cg(g, A_m)
+    want = r0.copy()
+    O.mul_ket(want, oop, b, alpha, beta)
+    r = Q.Ket(bl, r0, dtype)
+    Q.mul_(r, qop, Q.Ket(br, b, dtype), alpha, beta)
+    assert rel(r.to_numpy(), want) < tol, "ket"
+    # Bra
+    a = cg(g, A_m)
+    r0 = cg(g, A_n)
+    want = r0.copy()
+    O.mul_bra(want, a, oop, alpha, beta)
+    r = Q.Bra(br, r0, dtype)
+    Q.mul_(r, Q.Bra(bl, a, dtype), qop, alpha, beta)
+    assert rel(r.to_numpy(), want) < tol, "bra"
+    # Operator left
+    cb = gb(ncols)
+    B = cg(g, A_n, ncols)
+    R0 = cg(g, A_m, ncols)
+    want = R0.copy()
+    O.mul_op_left(want, oop, B, alpha, beta)
+    R = Q.Operator(bl, cb, R0, dtype)
+    Q.mul_(R, qop, Q.Operator(br, cb, B, dtype), alpha, beta)
+    assert rel(R.to_numpy(), want) < tol, "op-left"
+    # Operator right
+    B = cg(g, ncols + 1, A_m)
+    R0 = cg(g, ncols + 1, A_n)
+    want = R0.copy()
+    O.mul_op_right(want, B, oop, alpha, beta)
+    rb = gb(ncols + 1)
+    R = Q.Operator(rb, br, R0, dtype)
+    Q.mul_(R, Q.Operator(rb, bl, B, dtype), qop, alpha, beta)
+    assert rel(R.to_numpy(), want) < tol, "op-right"
+
+
+# ------------------------------------------------------------------ sparse
+@pytest.mark.parametrize("m,n,density", [(3, 5, 0.5), (50, 60, 0.3), (4, 4, 0.5)])
+@pytest.mark.parametrize("alpha,beta", AB)
+@pytest.mark.parametrize("adj", [False, True])
+def test_sparse_operator(m, n, density, alpha, beta, adj):
+    g = np.random.default_rng(100 + m)
+    oop = O.OSparse.from_scipy(rs(g, m, n, density))
+    if adj:
+        oop = O.dagger(oop)
+    run_all_paths(oop, to_q(oop), g, alpha, beta, ncols=7)
+
+
+def test_sparse_operator_c32():
+    g = np.random.default_rng(5)
+    oop = O.OSparse.from_scipy(rs(g, 50, 60, 0.3))
+    run_all_paths(oop, to_q(oop), g, 1.5, 2.1, np.complex64, TOL32, ncols=7)
+
+
+# -------------------------------------------------------------- LazyTensor
+@pytest.mark.parametrize("case", range(len(LT_CASES)))
+@pytest.mark.parametrize("alpha,beta", AB)
+def test_lazytensor(case, alpha, beta):
+    shl, shr, idx, kinds = LT_CASES[case]
+    g = np.random.default_rng(10 + case)
+    lt = rand_lt(g, shl, shr, idx, kinds)
+    run_all_paths(lt, to_q(lt), g, alpha, beta)
+
+
+@pytest.mark.parametrize("case", [0, 1, 4, 7])
+def test_lazytensor_c32(case):
+    shl, shr, idx, kinds = LT_CASES[case]
+    g = np.random.default_rng(10 + case)
+    lt = rand_lt(g, shl, shr, idx, kinds)
+    run_all_paths(lt, to_q(lt), g, 1.5, 2.1, np.complex64, TOL32)
+
+
+def test_lazytensor_alpha_zero_beta_zero_nan():
+    g = np.random.default_rng(21)
+    lt = rand_lt(g, (2, 3), (2, 3), (1, 2), ("sparse", "sparse"))
+    q = to_q(lt)
+    r = Q.Ket(q.basis_l, np.full(6, np.nan, complex))
+    Q.mul_(r, q, Q.Ket(q.basis_r, cg(g, 6)), 0.0, 0.0)
+    assert (r.to_numpy() == 0).all()
+    r = Q.Ket(q.basis_l, np.full(6, np.nan, complex))
+    Q.mul_(r, q, Q.Ket(q.basis_r, cg(g, 6)), 1.0, 0.0)
+    assert np.isfinite(r.to_numpy()).all()
+
+
+def test_lazytensor_larger_spin_chain_shapes():
+    # 12 qubits, mixed terms: exercises digit extraction on many sites
+    g = np.random.default_rng(22)
+    sh = (2,) * 12
+    lt = rand_lt(g, sh, sh, (1, 5, 12), ("sparse", "sparse", "sparse"))
+    run_all_paths(lt, to_q(lt), g, 0.3 + 0.1j, 0.5, ncols=3)
+
+
+# ---------------------------------------------------------------- LazySum
+def lazysum_case(g):
+    shl = shr = (2, 3, 2)
+    ops = [rand_lt(g, shl, shr, (1, 2), ("sparse", "sparse")),
+           rand_lt(g, shl, shr, (3,), ("dense",)),
+           rand_lt(g, shl, shr, (2, 3), ("sparse", "sparse")),
+           O.OSparse.from_scipy(rs(g, 12, 12, 0.3)),
+           O.OSparse.from_scipy(rs(g, 12, 12, 0.2))]
+    return O.OLazySum(12, 12, [0.1, 0.3, 1.2 - 0.4j, -0.7, 0.25j], ops)
+
+
+@pytest.mark.parametrize("mode", [Q.QOB_MODE_FUSED, Q.QOB_MODE_PLAIN])
+@pytest.mark.parametrize("alpha,beta", AB)
+def test_lazysum(mode, alpha, beta):
+    g = np.random.default_rng(30)
+    ls = lazysum_case(g)
+    ctx = Q.context()
+    ctx.set_mode(mode)
+    try:
+        run_all_paths(ls, to_q(ls), g, alpha, beta)
+    finally:
+        ctx.set_mode(Q.QOB_MODE_FUSED)
+
+
+def test_lazysum_zero_and_nan():
+    g = np.random.default_rng(31)
+    B = comp((2, 3))
+    z = Q.LazySum.zero(B, B)
+    r0 = cg(g, 6)
+    r = Q.Ket(B, r0)
+    Q.mul_(r, z, Q.Ket(B, cg(g, 6)), 1.5, 2.1)
+    assert rel(r.to_numpy(), 2.1 * r0) < TOL64
+    R = Q.Operator(B, B, np.full((6, 6), np.nan, complex))
+    Q.mul_(R, Q.Operator(B, B, cg(g, 6, 6)), z, 1.0, 0.0)
+    assert (R.to_numpy() == 0).all()
+    ls = lazysum_case(g)
+    q = to_q(ls)
+    R = Q.Operator(q.basis_l, q.basis_r, np.full((12, 12), np.nan, complex))
+    Bm = cg(g, 12, 12)
+    Q.mul_(R, q, Q.Operator(q.basis_r, q.basis_r, Bm), 1.0, 0.0)
+    want = np.zeros((12, 12), complex)
+    O.mul_op_left(want, ls, Bm, 1.0, 0.0)
+    assert rel(R.to_numpy(), want) < TOL64
+
+
+def test_lazysum_set_factors():
+    g = np.random.default_rng(32)
+    ls = lazysum_case(g)
+    q = to_q(ls)
+    b = cg(g, 12)
+    r = Q.Ket(q.basis_l, np.zeros(12, complex))
+    kb = Q.Ket(q.basis_r, b)
+    Q.mul_(r, q, kb)
+    new = [0.5, -1.0, 0.2, 0.3j, 1.0]
+    q.set_factors(new)
+    ls.factors = new
+    Q.mul_(r, q, kb)
+    want = np.zeros(12, complex)
+    O.mul_ket(want, ls, b)
+    assert rel(r.to_numpy(), want) < TOL64
+
+
+# ------------------------------------------------------------------ dense
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (complex(0.7, 1.5), complex(0.3, 2.1))])
+def test_dense_operator_paths(alpha, beta):
+    g = np.random.default_rng(40)
+    b1, b2, b3 = gb(37), gb(53), gb(41)
+    A = cg(g, 37, 53)
+    opq = Q.DenseOperator(b1, b2, A)
+    x = cg(g, 53)
+    r0 = cg(g, 37)
+    r = Q.Ket(b1, r0)
+    Q.mul_(r, opq, Q.Ket(b2, x), alpha, beta)
+    assert rel(r.to_numpy(), alpha * A @ x + beta * r0) < TOL64
+    y = cg(g, 37)
+    r0 = cg(g, 53)
+    r = Q.Bra(b2, r0)
+    Q.mul_(r, Q.Bra(b1, y), opq, alpha, beta)
+    assert rel(r.to_numpy(), alpha * A.T @ y + beta * r0) < TOL64
+    B = cg(g, 53, 41)
+    R0 = cg(g, 37, 41)
+    R = Q.Operator(b1, b3, R0)
+    Q.mul_(R, opq, Q.Operator(b2, b3, B), alpha, beta)
+    assert rel(R.to_numpy(), alpha * A @ B + beta * R0) < TOL64
+
+
+# ------------------------------------------------------------------ errors
+def test_dimension_mismatch_and_alias():
+    g = np.random.default_rng(50)
+    b1, b2, b3 = Q.NLevelBasis(2), Q.NLevelBasis(3), Q.NLevelBasis(4)
+    S = Q.SparseOperator(b2, b2, rs(g, 3, 3, 0.6))
+    with pytest.raises(Q.DimensionMismatch):
+        Q.mul_(Q.Ket(b1, cg(g, 2)), S, Q.Ket(b3, cg(g, 4)))
+    with pytest.raises(Q.DimensionMismatch):
+        Q.mul_(Q.Operator(b1, b1, cg(g, 2, 2)), S, Q.Operator(b3, b3, cg(g, 4, 4)))
+    k = Q.Ket(b2, cg(g, 3))
+    with pytest.raises(Q.ArgumentError):
+        Q.mul_(k, S, k)
+
+
+# --------------------------------------------------------------- reductions
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-13), (np.complex64, 1e-6)])
+def test_reductions(dtype, tol):
+    g = np.random.default_rng(60)
+    n = 1000
+    x, y = cg(g, n).astype(dtype), cg(g, n).astype(dtype)
+    b = gb(n)
+    kx, ky = Q.Ket(b, x), Q.Ket(b, y)
+    xd, yd = x.astype(np.complex128), y.astype(np.complex128)
+    assert abs(Q.norm(kx) - np.linalg.norm(xd)) < tol * np.linalg.norm(xd)
+    assert abs(Q.dot(kx, ky) - np.vdot(xd, yd)) < tol * np.linalg.norm(xd) * np.linalg.norm(yd)
+    A = cg(g, 70, 70).astype(dtype)
+    op = Q.Operator(gb(70), gb(70), A)
+    assert abs(Q.tr(op) - np.trace(A.astype(np.complex128))) < tol * np.abs(A).sum()
+    Q.normalize_(kx)
+    assert abs(np.linalg.norm(kx.to_numpy().astype(np.complex128)) - 1) < 10 * tol
+    Q.normalize_(op)
+    assert abs(np.trace(op.to_numpy().astype(np.complex128)) - 1) < 10 * tol
+
+
+def test_expect_matches_oracle():
+    g = np.random.default_rng(61)
+    lt = rand_lt(g, (3, 4), (3, 4), (1, 2), ("sparse", "sparse"))
+    q = to_q(lt)
+    psi = cg(g, 12)
+    want = O.expect_ket(lt, psi)
+    got = Q.expect(q, Q.Ket(q.basis_r, psi))
+    assert abs(got - want) < 1e-12 * abs(want)
+    rho = cg(g, 12, 12)
+    want = O.expect_op(lt, rho)
+    got = Q.expect(q, Q.Operator(q.basis_r, q.basis_r, rho))
+    assert abs(got - want) < 1e-12 * abs(want)
