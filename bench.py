@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--probe", action="store_true",
+                    help="only launch copy/left/right 5 times each (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU "
@@ -122,6 +124,22 @@ def main():
 
     def right():
         Q.mul_(outR, rho, H, 1.0, 0.0)
+
+    if args.probe:
+        # fixed launch sequence for counter collection (rocprofv3 --pmc):
+        # 5 x plain copy of N^2 elements, 5 x left, 5 x right
+        from qobhip import _lib
+        for _ in range(5):
+            _lib.call("qob_copy", ctx.h, 0, n * n, rho.data.ptr, outL.data.ptr, None)
+        ctx.synchronize()
+        for _ in range(5):
+            left()
+        ctx.synchronize()
+        for _ in range(5):
+            right()
+        ctx.synchronize()
+        log(f"bench: probe done (copy bytes per launch {32 * n * n})")
+        return
 
     log(f"bench: warmup {args.warmup}")
     for _ in range(args.warmup):

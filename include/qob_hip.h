@@ -178,12 +178,23 @@ qob_status qob_gemm(qob_ctx ctx, qob_dtype dt, int transa, int64_t m, int64_t n,
                     int64_t k, qob_c64 alpha, const void *A, const void *B,
                     qob_c64 beta, void *C, qob_stream s);
 
+/* ------------------------------------------------------------- tuning
+ * Launch-shape knobs read by the kernels (no counterpart in the reference:
+ * the reference has no device launch shapes).  Names: "sell_variant",
+ * "sell_cj", "sell_jchunk", "csc_rb", "csc_cc", "csc_remap".  The project's
+ * autotuner (qobhip/tune.py) sweeps them and persists a table; unknown names
+ * return QOB_EARG.  Settings are process-wide. */
+qob_status qob_tune_set(const char *name, int64_t value);
+qob_status qob_tune_get(const char *name, int64_t *value);
+
 /* -------------------------------------------- elementwise and reductions
  * Reductions accumulate in FP64 for both dtypes.  The host-result variants
  * synchronise the stream; the *_d variants write one qob_c64 / double to
  * device memory and stay stream-ordered (usable under stream capture). */
 /* x = a * x; a == 0 writes zeros (src/operators_lazysum.jl:179-186). */
 qob_status qob_scale(qob_ctx ctx, qob_dtype dt, int64_t n, qob_c64 a, void *x, qob_stream s);
+/* y = x (a plain streaming copy kernel; also the bandwidth probe). */
+qob_status qob_copy(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x, void *y, qob_stream s);
 /* y = a * x + y. */
 qob_status qob_axpy(qob_ctx ctx, qob_dtype dt, int64_t n, qob_c64 a,
                     const void *x, void *y, qob_stream s);

@@ -29,7 +29,7 @@ using cd = std::complex<double>;
 
 namespace qob {
 
-extern int g_sell_cj, g_csc_rb, g_csc_cc, g_csc_remap;
+extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -218,6 +218,8 @@ static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
     S.dev.m = m;
     S.dev.n = X.nc;
     S.dev.nslices = nslices;
+    S.dev.maxw = 0;
+    for (int k = 0; k < nslices; ++k) S.dev.maxw = std::max(S.dev.maxw, (off[k + 1] - off[k]) / 64);
     S.dev.slice_off = (const int32_t *)S.off.p;
     S.dev.col = (const int32_t *)S.col.p;
     S.dev.val = S.val.p;
@@ -1420,6 +1422,47 @@ qob_status qob_scale(qob_ctx ctx, qob_dtype dt, int64_t n, qob_c64 a, void *x, q
     if (!ctx) return fail(QOB_EARG, "null context");
     hipError_t e = launch_scale(dt, n, d2(a), x, strm(ctx, s));
     if (e != hipSuccess) return fail(QOB_EHIP, "scale: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_copy(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x, void *y, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    if (x == y) return fail(QOB_EALIAS, "copy with x == y");
+    hipError_t e = launch_copy(dt, n, x, y, strm(ctx, s));
+    if (e != hipSuccess) return fail(QOB_EHIP, "copy: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+static int *tune_slot(const char *name) {
+    static const struct {
+        const char *n;
+        int *p;
+    } tbl[] = {{"sell_variant", &g_sell_variant}, {"sell_cj", &g_sell_cj},     {"sell_jchunk", &g_sell_jchunk},
+               {"csc_rb", &g_csc_rb},             {"csc_cc", &g_csc_cc},       {"csc_remap", &g_csc_remap},
+               {"nt_store", &g_nt_store},         {"copy_variant", &g_copy_variant}};
+    for (auto &e : tbl)
+        if (name && strcmp(name, e.n) == 0) return e.p;
+    return nullptr;
+}
+
+qob_status qob_tune_set(const char *name, int64_t value) {
+    int *p = tune_slot(name);
+    if (!p) return fail(QOB_EARG, "unknown tunable '%s'", name ? name : "(null)");
+    if (value < 0 || value > (1 << 20)) return fail(QOB_EARG, "tunable value %lld out of range", (long long)value);
+    if (!strcmp(name, "csc_rb") && value != 64 && value != 128 && value != 256)
+        return fail(QOB_EARG, "csc_rb must be 64, 128 or 256");
+    if (!strcmp(name, "sell_cj") && value != 1 && value != 2 && value != 4 && value != 8)
+        return fail(QOB_EARG, "sell_cj must be 1, 2, 4 or 8");
+    if ((!strcmp(name, "csc_cc") || !strcmp(name, "sell_jchunk")) && value < 1)
+        return fail(QOB_EARG, "%s must be >= 1", name);
+    *p = (int)value;
+    return QOB_OK;
+}
+
+qob_status qob_tune_get(const char *name, int64_t *value) {
+    int *p = tune_slot(name);
+    if (!p || !value) return fail(QOB_EARG, "unknown tunable '%s'", name ? name : "(null)");
+    *value = *p;
     return QOB_OK;
 }
 

@@ -30,6 +30,28 @@ __global__ void __launch_bounds__(BS) k_scale(int64_t n, double2 a_, T *x) {
 }
 
 template <typename T>
+__global__ void __launch_bounds__(BS) k_copy(int64_t n, const T *__restrict__ x, T *__restrict__ y) {
+    for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) y[i] = x[i];
+}
+
+// 4 elements per lane in flight, non-temporal stores
+template <typename T>
+__global__ void __launch_bounds__(BS) k_copy4(int64_t n, const T *__restrict__ x, T *__restrict__ y) {
+    const int64_t base = blockIdx.x * (int64_t)BS * 4 + threadIdx.x;
+    T v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = base + (int64_t)k * BS;
+        if (i < n) v[k] = x[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = base + (int64_t)k * BS;
+        if (i < n) st_stream(y + i, v[k]);
+    }
+}
+
+template <typename T>
 __global__ void __launch_bounds__(BS) k_axpy(int64_t n, double2 a_, const T *__restrict__ x, T *__restrict__ y) {
     const T a = from_d2<T>(a_);
     for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
@@ -181,6 +203,8 @@ hipError_t launch_embed(qob_dtype dt, const void *in, int64_t ri, int64_t ci, vo
     return hipGetLastError();
 }
 
+int g_copy_variant = 1;
+
 int reduce_blocks(int64_t n) {
     int64_t g = (n + BS - 1) / BS;
     if (g > 1024) g = 1024;
@@ -192,6 +216,20 @@ hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t
     if (n <= 0) return hipSuccess;
     if (dt == QOB_C64) hipLaunchKernelGGL(k_scale<double2>, grid_for(n), BS, 0, s, n, a, (double2 *)x);
     else hipLaunchKernelGGL(k_scale<float2>, grid_for(n), BS, 0, s, n, a, (float2 *)x);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(qob_dtype dt, int64_t n, const void *x, void *y, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (g_copy_variant == 1 && (n + 4 * BS - 1) / (4 * BS) < 0x7fffffff) {
+        const unsigned g4 = (unsigned)((n + 4 * BS - 1) / (4 * BS));
+        if (dt == QOB_C64) hipLaunchKernelGGL(k_copy4<double2>, g4, BS, 0, s, n, (const double2 *)x, (double2 *)y);
+        else hipLaunchKernelGGL(k_copy4<float2>, g4, BS, 0, s, n, (const float2 *)x, (float2 *)y);
+        return hipGetLastError();
+    }
+    const unsigned g = (unsigned)std::min<int64_t>((n + BS - 1) / BS, 1 << 20);
+    if (dt == QOB_C64) hipLaunchKernelGGL(k_copy<double2>, g, BS, 0, s, n, (const double2 *)x, (double2 *)y);
+    else hipLaunchKernelGGL(k_copy<float2>, g, BS, 0, s, n, (const float2 *)x, (float2 *)y);
     return hipGetLastError();
 }
 

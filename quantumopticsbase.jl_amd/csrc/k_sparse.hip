@@ -75,7 +75,62 @@ __global__ void __launch_bounds__(256) k_sell_left(int nslices, int64_t m, const
         for (int jj = 0; jj < CJ; ++jj) {
             if (j0 + jj < cols) {
                 T *o = R + row + (j0 + jj) * ldr;
-                *o = axpby(alpha, acc[jj], beta, o, beta_zero != 0);
+                st_out(o, axpby(alpha, acc[jj], beta, o, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
+            }
+        }
+    }
+}
+
+// ---- left, register-resident rows: each lane loads its row's (col, val)
+// entries once and then walks a chunk of columns of B, UNR at a time, so the
+// sparse tables are read once per workgroup instead of once per column
+// group.  Logical block = (column chunk cg, slice group sg), slice group
+// fastest.
+template <typename T, int MAXW, int UNR>
+__global__ void __launch_bounds__(256) k_sell_left_rr(int nslices, int64_t m, const int32_t *__restrict__ slice_off,
+                                                      const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                      const T *__restrict__ B, int64_t ldb, int64_t cols,
+                                                      T *__restrict__ R, int64_t ldr, double2 alpha_,
+                                                      double2 beta_, int beta_zero, uint32_t nsg, int jchunk) {
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = xcd_remap(blockIdx.x, G);
+    const uint32_t sg = lb % nsg, cg = lb / nsg;
+    const int slice = (int)(sg * 4 + (threadIdx.x >> 6));
+    const int r = threadIdx.x & 63;
+    if (slice >= nslices) return;
+    const int64_t row = (int64_t)slice * 64 + r;
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+    const int32_t o0 = slice_off[slice];
+    const int w = (slice_off[slice + 1] - o0) >> 6;
+    int32_t c[MAXW];
+    T v[MAXW];
+#pragma unroll
+    for (int p = 0; p < MAXW; ++p) {
+        c[p] = (p < w) ? col[o0 + p * 64 + r] : -1;
+        v[p] = (p < w) ? val[o0 + p * 64 + r] : czero<T>();
+    }
+    const int64_t j0 = (int64_t)cg * jchunk;
+    const int64_t j1 = (j0 + jchunk < cols) ? j0 + jchunk : cols;
+    const bool live = row < m;
+    for (int64_t j = j0; j < j1; j += UNR) {
+        T acc[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) acc[u] = czero<T>();
+#pragma unroll
+        for (int p = 0; p < MAXW; ++p) {
+            if (c[p] >= 0) {
+#pragma unroll
+                for (int u = 0; u < UNR; ++u)
+                    if (j + u < j1) acc[u] = cfma(v[p], B[(int64_t)c[p] + (j + u) * ldb], acc[u]);
+            }
+        }
+        if (live) {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                if (j + u < j1) {
+                    T *o = R + row + (j + u) * ldr;
+                    st_out(o, axpby(alpha, acc[u], beta, o, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
+                }
             }
         }
     }
@@ -104,7 +159,7 @@ __global__ void __launch_bounds__(RB) k_csc_right(int64_t rows, int64_t n, const
                 acc = cfma(val[p], B[i + (int64_t)rowind[p] * rows], acc);
             }
             T *o = R + i + c * rows;
-            *o = axpby(alpha, acc, beta, o, beta_zero != 0);
+            st_out(o, axpby(alpha, acc, beta, o, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
         }
     }
 }
@@ -123,22 +178,48 @@ __global__ void __launch_bounds__(256) k_csc_right_thin(int64_t rows, int64_t n,
         T acc = czero<T>();
         for (int32_t p = p0; p < p1; ++p) acc = cfma(val[p], B[i + (int64_t)rowind[p] * rows], acc);
         T *o = R + i + c * rows;
-        *o = axpby(alpha, acc, beta, o, beta_zero != 0);
+        st_out(o, axpby(alpha, acc, beta, o, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
     }
 }
 
 }  // namespace
 
 // Tunables (set once from the host; defaults chosen from measurements).
+int g_sell_variant = 1;  // 0: column groups of CJ, 1: register-resident rows
 int g_sell_cj = 4;
-int g_csc_rb = 256;
-int g_csc_cc = 8;
-int g_csc_remap = 0;
+int g_sell_jchunk = 8;
+int g_csc_rb = 64;
+int g_csc_cc = 2;
+int g_csc_remap = 1;
+int g_nt_store = 1;
+
+template <typename T>
+static void sell_rr(const SellDev &X, const void *B, int64_t cols, void *R, double2 alpha, double2 beta, int bz,
+                    hipStream_t s, int maxw, uint32_t nsg, uint32_t G, int jchunk) {
+#define QOB_RR(MW, U)                                                                                          \
+    hipLaunchKernelGGL((k_sell_left_rr<T, MW, U>), dim3(G), dim3(256), 0, s, X.nslices, X.m, X.slice_off, X.col, \
+                       (const T *)X.val, (const T *)B, X.n, cols, (T *)R, X.m, alpha, beta, bz, nsg, jchunk)
+    if (maxw <= 2) QOB_RR(2, 4);
+    else if (maxw <= 4) QOB_RR(4, 4);
+    else if (maxw <= 8) QOB_RR(8, 2);
+    else QOB_RR(16, 1);
+#undef QOB_RR
+}
 
 hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols, void *R,
                             double2 alpha, double2 beta, hipStream_t s) {
     if (X.m == 0 || cols == 0) return hipSuccess;
-    const int bz = (beta.x == 0.0 && beta.y == 0.0);
+    const int bz = (beta.x == 0.0 && beta.y == 0.0) | (g_nt_store << 1);
+    if (g_sell_variant == 1 && X.maxw <= 16) {
+        const uint32_t nsg = (uint32_t)((X.nslices + 3) / 4);
+        const int jchunk = (int)std::max<int64_t>(1, std::min<int64_t>(g_sell_jchunk, cols));
+        const uint64_t ncg = (uint64_t)((cols + jchunk - 1) / jchunk);
+        const uint64_t G = (uint64_t)nsg * ncg;
+        if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
+        if (dt == QOB_C64) sell_rr<double2>(X, B, cols, R, alpha, beta, bz, s, X.maxw, nsg, (uint32_t)G, jchunk);
+        else sell_rr<float2>(X, B, cols, R, alpha, beta, bz, s, X.maxw, nsg, (uint32_t)G, jchunk);
+        return hipGetLastError();
+    }
     const uint32_t nsg = (uint32_t)((X.nslices + 3) / 4);
     const int cj = (cols == 1) ? 1 : g_sell_cj;
     const uint32_t ncg = (uint32_t)((cols + cj - 1) / cj);
@@ -166,7 +247,7 @@ hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64
 hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_t rows, void *R,
                             double2 alpha, double2 beta, hipStream_t s) {
     if (X.n == 0 || rows == 0) return hipSuccess;
-    const int bz = (beta.x == 0.0 && beta.y == 0.0);
+    const int bz = (beta.x == 0.0 && beta.y == 0.0) | (g_nt_store << 1);
     if (rows < 32) {
         const uint32_t g = (uint32_t)((X.n + 255) / 256);
         if (dt == QOB_C64)

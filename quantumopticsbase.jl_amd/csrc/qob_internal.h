@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstddef>
+#include <algorithm>
 
 #include "../../include/qob_hip.h"
 
@@ -48,6 +49,18 @@ template <typename T> __device__ __forceinline__ T axpby(T alpha, T acc, T beta,
     T r = cmul(alpha, acc);
     if (!beta_zero) r = cfma(beta, *old, r);
     return r;
+}
+
+// Streaming store of a result element that is not read again by the same
+// kernel: the non-temporal hint keeps it from evicting input lines that
+// later gathers of the same launch still need.
+template <typename T> __device__ __forceinline__ void st_stream(T *p, T v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+template <typename T> __device__ __forceinline__ void st_out(T *p, T v, bool nt) {
+    if (nt) st_stream(p, v);
+    else *p = v;
 }
 
 // ------------------------------------------------------ fast integer divide
@@ -118,6 +131,7 @@ struct GatherPlanDev {
 
 // elementwise (k_blas1.hip)
 hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t s);
+hipError_t launch_copy(qob_dtype dt, int64_t n, const void *x, void *y, hipStream_t s);
 hipError_t launch_axpy(qob_dtype dt, int64_t n, double2 a, const void *x, void *y, hipStream_t s);
 hipError_t launch_axpby_copy(qob_dtype dt, int64_t n, double2 a, const void *x, double2 b, void *y, hipStream_t s);
 hipError_t launch_embed(qob_dtype dt, const void *in, int64_t ri, int64_t ci, void *out, int64_t ro, int64_t co,
@@ -139,6 +153,7 @@ hipError_t launch_scale_by_inv_trace(qob_dtype dt, int64_t n, int64_t ld, const 
 struct SellDev {
     int64_t m = 0, n = 0;
     int nslices = 0;
+    int maxw = 0;                         // widest slice
     const int32_t *slice_off = nullptr;  // nslices+1 (in entries)
     const int32_t *col = nullptr;
     const void *val = nullptr;            // CF64 or CF32

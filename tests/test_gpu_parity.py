@@ -36,7 +36,7 @@ def to_q(op, bl=None, br=None):
     if isinstance(op, O.OSparse):
         M = sp.csc_matrix((op.nzval, op.rowval - 1, op.colptr - 1), shape=(op.m, op.n))
         if op.adj:
-            return Q.dagger(Q.SparseOperator(br or gb(op.n), bl or gb(op.m), M))
+            return Q.dagger(Q.SparseOperator(br or gb(op.m), bl or gb(op.n), M))
         return Q.SparseOperator(bl or gb(op.m), br or gb(op.n), M)
     if isinstance(op, O.ODense):
         return Q.DenseOperator(bl or gb(op.shape[0]), br or gb(op.shape[1]), op.mat())
