@@ -178,6 +178,36 @@ qob_status qob_gemm(qob_ctx ctx, qob_dtype dt, int transa, int64_t m, int64_t n,
                     int64_t k, qob_c64 alpha, const void *A, const void *B,
                     qob_c64 beta, void *C, qob_stream s);
 
+/* ---------------------------------------------------- batches and graphs
+ * No counterpart in the reference (it applies one operator per mul! call):
+ * many independent Kets of different lengths 2^L_j, each with its own
+ * LazySum / LazyTensor of qubit terms with monomial factors (Pauli, sigma_+-,
+ * number, projectors), applied in ONE persistent launch.  Buffers are bound
+ * to one of QOB_BATCH_SLOTS slots once (a device pointer table), so applies
+ * are pure kernel launches and can be captured into a HIP graph. */
+typedef struct qob_batch_s *qob_batch;
+typedef struct qob_graph_s *qob_graph;
+#define QOB_BATCH_SLOTS 4
+/* QOB_EUNSUPPORTED if an operator is not a qubit sum of monomial factors. */
+qob_status qob_batch_create(qob_ctx ctx, int n, const qob_op *ops, qob_dtype dt, qob_batch *out);
+/* Bind in[j] / out[j] (device pointers, 2^L_j elements each) to a slot. */
+qob_status qob_batch_bind(qob_ctx ctx, qob_batch b, int slot, void *const *in, void *const *out);
+/* For every j: out_j = alpha * op_j * in_j + gamma * in_j + beta * out_j
+ * (gamma fuses the Ket update of a time step; in_j must not alias out_j). */
+qob_status qob_batch_apply(qob_ctx ctx, qob_batch b, int slot, qob_c64 alpha, qob_c64 beta, qob_c64 gamma,
+                           qob_stream s);
+/* For every j: out_j += a * in_j (the axpy of a time step). */
+qob_status qob_batch_axpy(qob_ctx ctx, qob_batch b, int slot, qob_c64 a, qob_stream s);
+/* Total number of basis states over the batch, and the number of states. */
+qob_status qob_batch_info(qob_batch b, int64_t *total_elements, int *nstates);
+qob_status qob_batch_destroy(qob_batch b);
+/* Stream capture of any sequence of library calls into a HIP graph.  Every
+ * captured call must have run once before (plans and workspaces exist). */
+qob_status qob_graph_begin(qob_ctx ctx, qob_stream s);
+qob_status qob_graph_end(qob_ctx ctx, qob_stream s, qob_graph *out);
+qob_status qob_graph_launch(qob_ctx ctx, qob_graph g, qob_stream s);
+qob_status qob_graph_destroy(qob_graph g);
+
 /* ------------------------------------------------------------- tuning
  * Launch-shape knobs read by the kernels (no counterpart in the reference:
  * the reference has no device launch shapes).  Names: "sell_variant",

@@ -14,6 +14,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
@@ -30,6 +31,8 @@ using cd = std::complex<double>;
 namespace qob {
 
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
+int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
+extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -347,6 +350,7 @@ struct OpNode {
     std::mutex mu;
     std::map<std::string, std::shared_ptr<GatherPlan>> gplans;
     std::map<std::string, std::shared_ptr<SparseData>> sums;
+    std::map<std::string, std::shared_ptr<struct QubitPlan>> qplans;
     uint64_t version = 0;  // bumped when coefficients change
 };
 
@@ -684,6 +688,225 @@ static qob_status lazy_plan(OpNode &owner, const std::string &key, const std::ve
     return QOB_OK;
 }
 
+// ------------------------------------------------------- qubit programs
+// A sum of tensor-product terms on qubits whose factors are monomial 2x2
+// matrices maps basis state J to J ^ F_t.  Terms are grouped by F and each
+// group's weight w_F(J) = sum_t c_t prod_s v_{t,s}[bit_s(J)] is stored as a
+// constant or as tables over windows of <= 8 bits (program layout in
+// k_qubit.hip).  Returns false when a term does not fit this form.
+struct QubitProgHost {
+    std::vector<int32_t> prog;     // flips inside a tile (all flips when prog_hi is empty)
+    std::vector<int32_t> prog_hi;  // flips of bits >= tb, for the strided second pass
+    std::vector<cd> vals;
+};
+
+// tb: tile bits of the launch; states longer than a tile (and at most 2 tb
+// bits) get their high-bit flips in a separate program.
+static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitProgHost &Q, int tb) {
+    struct QT {
+        cd coef;
+        uint32_t F;
+        std::vector<int> sites;
+        std::vector<std::array<cd, 2>> v;
+    };
+    std::vector<QT> qts;
+    for (const TermSpec &t : terms) {
+        QT q;
+        q.coef = t.coef;
+        q.F = 0;
+        bool zero = false;
+        for (size_t i = 0; i < t.sites.size(); ++i) {
+            const RowLists &X = *t.X[i];
+            if (X.nr != 2 || X.nc != 2) return false;
+            int flip = -1;
+            std::array<cd, 2> v{cd(0, 0), cd(0, 0)};
+            for (int j = 0; j < 2; ++j) {
+                const int n = X.ptr[j + 1] - X.ptr[j];
+                if (n > 1) return false;
+                if (n == 1) {
+                    const int k = X.idx[X.ptr[j]];
+                    const int f = (k != j) ? 1 : 0;
+                    if (flip >= 0 && flip != f) return false;
+                    flip = f;
+                    v[j] = X.val[X.ptr[j]];
+                }
+            }
+            if (flip < 0) {
+                zero = true;  // factor with no entries: the term vanishes
+                break;
+            }
+            q.F |= (uint32_t)flip << t.sites[i];
+            q.sites.push_back(t.sites[i]);
+            q.v.push_back(v);
+        }
+        if (!zero) qts.push_back(std::move(q));
+    }
+    std::map<uint32_t, std::vector<const QT *>> groups;
+    for (const QT &q : qts) groups[q.F].push_back(&q);
+    Q.prog.clear();
+    Q.prog_hi.clear();
+    Q.vals.clear();
+    const bool split = nbits > tb && nbits <= 2 * tb;
+    int nlo = 0, nhi = 0;
+    for (auto &gq : groups) ((split && (gq.first >> tb) != 0) ? nhi : nlo)++;
+    Q.prog.push_back(nlo);
+    if (nhi) Q.prog_hi.push_back(nhi);
+    for (auto &gq : groups) {
+        std::vector<int32_t> &P = (split && (gq.first >> tb) != 0) ? Q.prog_hi : Q.prog;
+        // windows of width <= 8 starting at multiples of 6 where possible
+        std::map<int, std::vector<cd>> win;
+        cd constant(0, 0);
+        bool has_const = false;
+        for (const QT *q : gq.second) {
+            if (q->sites.empty()) {
+                constant += q->coef;
+                has_const = true;
+                continue;
+            }
+            const int m = *std::min_element(q->sites.begin(), q->sites.end());
+            const int M = *std::max_element(q->sites.begin(), q->sites.end());
+            if (M - m >= 8) return false;
+            int ws = (m / 6) * 6;
+            if (M >= ws + 8) ws = m;
+            const int nb = std::min(8, nbits - ws);
+            auto &tab = win[ws];
+            if (tab.empty()) tab.assign((size_t)1 << nb, cd(0, 0));
+            for (int idx = 0; idx < (1 << nb); ++idx) {
+                cd w = q->coef;
+                for (size_t i = 0; i < q->sites.size(); ++i) w *= q->v[i][(idx >> (q->sites[i] - ws)) & 1];
+                tab[idx] += w;
+            }
+        }
+        // fold windows whose table is constant into the constant part
+        for (auto it = win.begin(); it != win.end();) {
+            bool c = true;
+            for (auto &z : it->second) c = c && (z == it->second[0]);
+            if (c) {
+                constant += it->second[0];
+                has_const = true;
+                it = win.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        P.push_back((int32_t)gq.first);
+        if (win.empty()) {
+            P.push_back(0);
+            P.push_back((int32_t)Q.vals.size());
+            Q.vals.push_back(constant);
+            continue;
+        }
+        P.push_back((int32_t)win.size());
+        P.push_back(0);
+        bool first = true;
+        for (auto &w : win) {
+            const int nb = std::min(8, nbits - w.first);
+            P.push_back(w.first);
+            P.push_back(nb);
+            P.push_back((int32_t)Q.vals.size());
+            for (auto &z : w.second) Q.vals.push_back(first && has_const ? z + constant : z);
+            first = false;
+        }
+    }
+    if (Q.vals.empty()) Q.vals.push_back(cd(0, 0));
+    return true;
+}
+
+struct QubitPlan {
+    bool ok = false;
+    int nbits = 0;
+    int tbits = 12;
+    int32_t hi_off = -1;  // offset of the high-bit program in prog, or -1
+    DevBuf prog, vals;
+};
+
+// Shift the value offsets of a compiled qubit program by vb.
+static void rebase_qubit_prog(std::vector<int32_t> &p, int32_t vb) {
+    if (p.empty()) return;
+    size_t i = 1;
+    for (int g = 0; g < p[0]; ++g) {
+        const int nwin = p[i + 1];
+        if (nwin == 0) p[i + 2] += vb;
+        i += 3;
+        for (int k = 0; k < nwin; ++k, i += 3) p[i + 2] += vb;
+    }
+}
+
+// Terms of a LazySum / LazyTensor as qubit terms (trans: transposed factors).
+static bool qubit_terms(OpNode &op, bool trans, std::vector<TermSpec> &specs, std::vector<std::vector<RowLists>> &store,
+                        int &nbits) {
+    std::vector<OpNode *> lts;
+    std::vector<cd> coefs;
+    if (op.kind == K_LT) {
+        lts.push_back(&op);
+        coefs.push_back(cd(1, 0));
+    } else if (op.kind == K_LS) {
+        for (size_t t = 0; t < op.terms.size(); ++t) {
+            lts.push_back(op.terms[t].get());
+            coefs.push_back(op.coef[t]);
+        }
+    } else {
+        return false;
+    }
+    if (lts.empty()) return false;
+    for (OpNode *lt : lts) {
+        if (!lt_pure_sparse(*lt) || lt->shl != lt->shr) return false;
+        for (int64_t d : lt->shl)
+            if (d != 2) return false;
+    }
+    nbits = (int)lts[0]->shl.size();
+    if (nbits < 1 || nbits > 30) return false;
+    store.assign(lts.size(), {});
+    for (size_t t = 0; t < lts.size(); ++t) {
+        const OpNode &lt = *lts[t];
+        TermSpec ts;
+        ts.coef = coefs[t] * lt.factor;
+        for (size_t i = 0; i < lt.fop.size(); ++i)
+            if (lt.fop[i]->kind != K_EYE) store[t].push_back(factor_rows(*lt.fop[i], trans));
+        size_t k2 = 0;
+        for (size_t i = 0; i < lt.fop.size(); ++i) {
+            if (lt.fop[i]->kind == K_EYE) continue;
+            ts.sites.push_back(lt.fsite[i]);
+            ts.X.push_back(&store[t][k2++]);
+        }
+        specs.push_back(std::move(ts));
+    }
+    return true;
+}
+
+int g_qubit_enable = 1;
+extern int g_qubit_tbits;
+
+static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, std::shared_ptr<QubitPlan> &out) {
+    std::lock_guard<std::mutex> g(op.mu);
+    const int tb = g_qubit_tbits;
+    const std::string k = "qb:" + std::to_string(op.version) + (trans ? ":T:" : ":N:") + std::to_string((int)dt) +
+                          ":" + std::to_string(tb);
+    auto it = op.qplans.find(k);
+    if (it == op.qplans.end()) {
+        auto P = std::make_shared<QubitPlan>();
+        std::vector<TermSpec> specs;
+        std::vector<std::vector<RowLists>> store;
+        int nbits = 0;
+        QubitProgHost H;
+        if (qubit_terms(op, trans, specs, store, nbits) && compile_qubit(nbits, specs, H, tb)) {
+            std::vector<int32_t> all = H.prog;
+            if (!H.prog_hi.empty()) {
+                P->hi_off = (int32_t)all.size();
+                all.insert(all.end(), H.prog_hi.begin(), H.prog_hi.end());
+            }
+            QOB_TRY(P->prog.upload(all.data(), all.size() * 4));
+            QOB_TRY(upload_vals(P->vals, H.vals, dt));
+            P->nbits = nbits;
+            P->tbits = tb;
+            P->ok = true;
+        }
+        it = op.qplans.emplace(k, P).first;
+    }
+    out = it->second;
+    return QOB_OK;
+}
+
 // ------------------------------------------------------------------ rocBLAS
 static qob_status blas_handle(qob_ctx ctx, rocblas_handle &h) {
     if (!ctx->blas) {
@@ -916,6 +1139,29 @@ static qob_status apply_lt_chain(const Apply &ap, OpNode &op, const void *in, vo
     }
     const cd a = alpha * op.factor;
     const int64_t Din = prod(sh_in, 0, N), Dout = prod(sh_out, 0, N);
+    // All factors dense and square on a column that fits in LDS: one fused
+    // MFMA pass over the state instead of one pass per factor.
+    if (g_fused_lt && ap.lead == 1 && !ops.empty() && sh_in == sh_out && Din < (1 << 30)) {
+        FusedModes FM;
+        FM.D = (int)Din;
+        bool ok = (int)ops.size() <= FUSED_MAXMODES;
+        for (size_t i = 0; ok && i < ops.size(); ++i) {
+            const ChainOp &c = ops[i];
+            if (c.f->kind != K_DENSE || c.d_in != c.d_out) {
+                ok = false;
+                break;
+            }
+            FM.n1[i] = (int)prod(sh_in, 0, c.site);
+            FM.d[i] = (int)c.d_in;
+            QOB_TRY(dense_dev(*c.f->de, c.f->var ^ (ap.trans ? V_T : V_N), ap.dt, FM.A[i]));
+        }
+        FM.nmodes = (int)ops.size();
+        if (ok && fused_supported(ap.dt, FM)) {
+            hipError_t e = launch_lt_fused(ap.dt, FM, in, out, ap.trail, d2(a), d2(beta), ap.s);
+            if (e != hipSuccess) return fail(QOB_EHIP, "fused LazyTensor launch: %s", hipGetErrorString(e));
+            return QOB_OK;
+        }
+    }
     if (ops.empty()) {
         // n_ops == 0: result = alpha * b + beta * result (:453-458)
         if (Din != Dout) return fail(QOB_EDIM, "LazyTensor without factors on unequal dimensions");
@@ -947,9 +1193,38 @@ static qob_status apply_lt_chain(const Apply &ap, OpNode &op, const void *in, vo
     return chain_step(ap, op, "lt", ops.back(), cur, t1, out, cd(1, 0), beta);
 }
 
+// Qubit Pauli-type sums on Kets / columns of rho (k_qubit.hip).  Sets done
+// when the operator qualified and the launch was issued.
+static qob_status apply_qubit(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta, bool &done) {
+    done = false;
+    if (!g_qubit_enable || ap.lead != 1) return QOB_OK;
+    std::shared_ptr<QubitPlan> P;
+    QOB_TRY(qubit_plan(op, ap.trans, ap.dt, P));
+    if (!P->ok) return QOB_OK;
+    QubitStateDev S;
+    S.nbits = P->nbits;
+    S.prog_off = 0;
+    S.prog_hi_off = P->hi_off;
+    S.in = in;
+    S.out = out;
+    hipError_t e = hipSuccess;
+    if (P->hi_off >= 0)  // high-bit flips first: out = alpha * high + beta * out
+        e = launch_qubit_hi_cols(ap.dt, S, ap.trail, P->tbits, (const int32_t *)P->prog.p, P->vals.p, d2(alpha),
+                                 d2(beta), ap.s);
+    if (e == hipSuccess)
+        e = launch_qubit_cols(ap.dt, S, ap.trail, P->tbits, (const int32_t *)P->prog.p, P->vals.p, d2(alpha),
+                              d2(beta), make_double2(0, 0), ap.s);
+    if (e != hipSuccess) return fail(QOB_EHIP, "qubit launch: %s", hipGetErrorString(e));
+    done = true;
+    return QOB_OK;
+}
+
 static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
     const int64_t dout = op_out_dim(op, ap.trans);
     if (is_zero(alpha)) return apply_scale(ap, dout, out, beta);  // :540
+    bool done = false;
+    QOB_TRY(apply_qubit(ap, op, in, out, alpha, beta, done));
+    if (done) return QOB_OK;
     if (lt_pure_sparse(op)) {
         // _mul_puresparse! -> _gemm_puresparse (:729-751)
         GatherPlanDev P;
@@ -970,6 +1245,11 @@ static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *ou
         for (size_t t = 0; t < op.terms.size(); ++t)
             QOB_TRY(apply_op(ap, *op.terms[t], in, out, alpha * op.coef[t], t == 0 ? beta : cd(1, 0)));
         return QOB_OK;
+    }
+    {
+        bool done = false;
+        QOB_TRY(apply_qubit(ap, op, in, out, alpha, beta, done));
+        if (done) return QOB_OK;
     }
     std::vector<OpNode *> lts;
     std::vector<cd> lcoef;
@@ -1334,6 +1614,7 @@ qob_status qob_lazysum_set_factors(qob_ctx ctx, qob_op sum, const qob_c64 *facto
     n.version++;
     n.gplans.clear();
     n.sums.clear();
+    n.qplans.clear();
     return QOB_OK;
 }
 
@@ -1439,7 +1720,10 @@ static int *tune_slot(const char *name) {
         int *p;
     } tbl[] = {{"sell_variant", &g_sell_variant}, {"sell_cj", &g_sell_cj},     {"sell_jchunk", &g_sell_jchunk},
                {"csc_rb", &g_csc_rb},             {"csc_cc", &g_csc_cc},       {"csc_remap", &g_csc_remap},
-               {"nt_store", &g_nt_store},         {"copy_variant", &g_copy_variant}};
+               {"nt_store", &g_nt_store},         {"copy_variant", &g_copy_variant},
+               {"fused_lt", &g_fused_lt},         {"qubit_enable", &g_qubit_enable},
+               {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
+               {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -1453,6 +1737,9 @@ qob_status qob_tune_set(const char *name, int64_t value) {
         return fail(QOB_EARG, "csc_rb must be 64, 128 or 256");
     if (!strcmp(name, "sell_cj") && value != 1 && value != 2 && value != 4 && value != 8)
         return fail(QOB_EARG, "sell_cj must be 1, 2, 4 or 8");
+    if (!strcmp(name, "qubit_tbits") && (value < 10 || value > 13)) return fail(QOB_EARG, "qubit_tbits must be in 10..13");
+    if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
+        return fail(QOB_EARG, "qubit_nth must be 256, 512 or 1024");
     if ((!strcmp(name, "csc_cc") || !strcmp(name, "sell_jchunk")) && value < 1)
         return fail(QOB_EARG, "%s must be >= 1", name);
     *p = (int)value;
@@ -1596,6 +1883,177 @@ qob_status qob_expect_op(qob_ctx ctx, qob_dtype dt, qob_op a, const void *rho, i
     Apply ap{ctx, st, dt, 1, n, false};
     QOB_TRY(apply_op(ap, op, rho, tmp, cd(1, 0), cd(0, 0)));
     return qob_trace(ctx, dt, n, tmp, out, s);
+}
+
+}  // extern "C"
+
+// ============================================================ batches
+struct qob_batch_s {
+    qob_ctx ctx = nullptr;
+    qob_dtype dt = QOB_C64;
+    int n = 0;
+    int tbits = 12;
+    int nitems = 0, nitems_hi = 0;
+    int64_t total = 0;
+    std::vector<int> nbits;
+    std::vector<int32_t> prog_off, prog_hi_off;
+    qob::DevBuf prog, vals, items, items_hi;
+    qob::DevBuf slot[QOB_BATCH_SLOTS];
+    bool bound[QOB_BATCH_SLOTS] = {false, false, false, false};
+};
+
+struct qob_graph_s {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+};
+
+extern "C" {
+
+qob_status qob_batch_create(qob_ctx ctx, int n, const qob_op *ops, qob_dtype dt, qob_batch *out) {
+    if (!ctx || !out || n < 1 || !ops) return fail(QOB_EARG, "bad arguments");
+    if (dt != QOB_C64 && dt != QOB_C32) return fail(QOB_EARG, "bad dtype");
+    std::unique_ptr<qob_batch_s> B(new qob_batch_s());
+    B->ctx = ctx;
+    B->dt = dt;
+    B->n = n;
+    B->tbits = g_qubit_tbits;
+    std::vector<int32_t> prog;
+    std::vector<cd> vals;
+    for (int j = 0; j < n; ++j) {
+        if (!ops[j]) return fail(QOB_EARG, "null operator %d", j);
+        std::vector<TermSpec> specs;
+        std::vector<std::vector<RowLists>> store;
+        int nb = 0;
+        QubitProgHost H;
+        if (!qubit_terms(*ops[j]->node, false, specs, store, nb) || !compile_qubit(nb, specs, H, B->tbits))
+            return fail(QOB_EUNSUPPORTED, "operator %d is not a qubit sum of monomial factors", j);
+        // rebase the value offsets of this state's programs into the shared pool
+        const int32_t vb = (int32_t)vals.size();
+        rebase_qubit_prog(H.prog, vb);
+        rebase_qubit_prog(H.prog_hi, vb);
+        B->prog_off.push_back((int32_t)prog.size());
+        prog.insert(prog.end(), H.prog.begin(), H.prog.end());
+        if (!H.prog_hi.empty()) {
+            B->prog_hi_off.push_back((int32_t)prog.size());
+            prog.insert(prog.end(), H.prog_hi.begin(), H.prog_hi.end());
+        } else {
+            B->prog_hi_off.push_back(-1);
+        }
+        vals.insert(vals.end(), H.vals.begin(), H.vals.end());
+        B->nbits.push_back(nb);
+        B->total += (int64_t)1 << nb;
+    }
+    // work items, largest states first
+    std::vector<int> order(n);
+    for (int j = 0; j < n; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->nbits[a] > B->nbits[b]; });
+    std::vector<QubitItemDev> items, items_hi;
+    for (int j : order) {
+        const int tb = std::min(B->tbits, B->nbits[j]);
+        const int64_t nt = (int64_t)1 << (B->nbits[j] - tb);
+        for (int64_t t = 0; t < nt; ++t) items.push_back({j, (int32_t)t});
+        if (B->prog_hi_off[j] >= 0)
+            for (int64_t t = 0; t < nt; ++t) items_hi.push_back({j, (int32_t)t});
+    }
+    if (items.size() >= (1ull << 31)) return fail(QOB_EUNSUPPORTED, "batch too large");
+    B->nitems = (int)items.size();
+    B->nitems_hi = (int)items_hi.size();
+    if (!items_hi.empty()) QOB_TRY(B->items_hi.upload(items_hi.data(), items_hi.size() * sizeof(QubitItemDev)));
+    QOB_TRY(B->prog.upload(prog.data(), prog.size() * 4));
+    QOB_TRY(upload_vals(B->vals, vals, dt));
+    QOB_TRY(B->items.upload(items.data(), items.size() * sizeof(QubitItemDev)));
+    *out = B.release();
+    return QOB_OK;
+}
+
+qob_status qob_batch_bind(qob_ctx ctx, qob_batch b, int slot, void *const *in, void *const *out) {
+    if (!ctx || !b || !in || !out) return fail(QOB_EARG, "null argument");
+    if (slot < 0 || slot >= QOB_BATCH_SLOTS) return fail(QOB_EARG, "slot %d out of range", slot);
+    std::vector<QubitStateDev> S(b->n);
+    for (int j = 0; j < b->n; ++j) {
+        if (!in[j] || !out[j]) return fail(QOB_EARG, "null buffer for state %d", j);
+        if (in[j] == out[j]) return fail(QOB_EALIAS, "state %d: output aliased with input", j);
+        S[j].nbits = b->nbits[j];
+        S[j].prog_off = b->prog_off[j];
+        S[j].prog_hi_off = b->prog_hi_off[j];
+        S[j].in = in[j];
+        S[j].out = out[j];
+    }
+    QOB_HIP(hipStreamSynchronize(ctx->stream));
+    QOB_TRY(b->slot[slot].upload(S.data(), S.size() * sizeof(QubitStateDev)));
+    b->bound[slot] = true;
+    return QOB_OK;
+}
+
+qob_status qob_batch_apply(qob_ctx ctx, qob_batch b, int slot, qob_c64 alpha, qob_c64 beta, qob_c64 gamma,
+                           qob_stream s) {
+    if (!ctx || !b) return fail(QOB_EARG, "null argument");
+    if (slot < 0 || slot >= QOB_BATCH_SLOTS || !b->bound[slot]) return fail(QOB_EARG, "slot %d not bound", slot);
+    hipError_t e = hipSuccess;
+    if (b->nitems_hi > 0)  // high-bit flips first: out = alpha * high + beta * out
+        e = launch_qubit_hi(b->dt, (const QubitStateDev *)b->slot[slot].p, (const QubitItemDev *)b->items_hi.p,
+                            b->nitems_hi, b->tbits, (const int32_t *)b->prog.p, b->vals.p, d2(alpha), d2(beta),
+                            strm(ctx, s));
+    if (e == hipSuccess)
+        e = launch_qubit(b->dt, (const QubitStateDev *)b->slot[slot].p, (const QubitItemDev *)b->items.p, b->nitems,
+                         b->tbits, (const int32_t *)b->prog.p, b->vals.p, d2(alpha), d2(beta), d2(gamma),
+                         strm(ctx, s));
+    if (e != hipSuccess) return fail(QOB_EHIP, "batch apply: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_batch_axpy(qob_ctx ctx, qob_batch b, int slot, qob_c64 a, qob_stream s) {
+    if (!ctx || !b) return fail(QOB_EARG, "null argument");
+    if (slot < 0 || slot >= QOB_BATCH_SLOTS || !b->bound[slot]) return fail(QOB_EARG, "slot %d not bound", slot);
+    hipError_t e = launch_qubit_axpy(b->dt, (const QubitStateDev *)b->slot[slot].p, (const QubitItemDev *)b->items.p,
+                                     b->nitems, b->tbits, d2(a), strm(ctx, s));
+    if (e != hipSuccess) return fail(QOB_EHIP, "batch axpy: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+qob_status qob_batch_info(qob_batch b, int64_t *total_elements, int *nstates) {
+    if (!b) return fail(QOB_EARG, "null batch");
+    if (total_elements) *total_elements = b->total;
+    if (nstates) *nstates = b->n;
+    return QOB_OK;
+}
+
+qob_status qob_batch_destroy(qob_batch b) {
+    if (b && b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return QOB_OK;
+}
+
+qob_status qob_graph_begin(qob_ctx ctx, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    QOB_HIP(hipStreamBeginCapture(strm(ctx, s), hipStreamCaptureModeThreadLocal));
+    return QOB_OK;
+}
+
+qob_status qob_graph_end(qob_ctx ctx, qob_stream s, qob_graph *out) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    std::unique_ptr<qob_graph_s> G(new qob_graph_s());
+    QOB_HIP(hipStreamEndCapture(strm(ctx, s), &G->g));
+    if (hipGraphInstantiate(&G->ex, G->g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(G->g);
+        return fail(QOB_EHIP, "hipGraphInstantiate failed");
+    }
+    *out = G.release();
+    return QOB_OK;
+}
+
+qob_status qob_graph_launch(qob_ctx ctx, qob_graph g, qob_stream s) {
+    if (!ctx || !g) return fail(QOB_EARG, "null argument");
+    QOB_HIP(hipGraphLaunch(g->ex, strm(ctx, s)));
+    return QOB_OK;
+}
+
+qob_status qob_graph_destroy(qob_graph g) {
+    if (!g) return QOB_OK;
+    if (g->ex) (void)hipGraphExecDestroy(g->ex);
+    if (g->g) (void)hipGraphDestroy(g->g);
+    delete g;
+    return QOB_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,268 @@
+// Fused dense LazyTensor application on MFMA: all mode products of a
+// column in one pass.
+//
+// Reference: _tp_sum_matmul! (src/operators_lazytensor.jl:443-488) applies
+// the factors one mode at a time, each _tp_matmul! a full pass over the
+// state through temporaries, with permutations for middle modes
+// (:333-404).  Here one workgroup owns one column (a Ket, or one column of a
+// density matrix for the left application) at a time:
+//   1. the column is loaded once into LDS as separate real and imaginary
+//      planes, padded by one element every 16 (conflict-free MFMA operand
+//      reads for every mode stride);
+//   2. for each factor A (d x d, d a multiple of 16) the mode product
+//      Y = A x_k X is a GEMM  (d x d) * (d x D/d): 16 x 16 output tiles on
+//      v_mfma_{f64,f32}_16x16x4, four real MFMAs per complex k-step
+//      (re*re - im*im, re*im + im*re), accumulators in registers; all waves
+//      finish reading before any tile is written back into the planes;
+//   3. the column is written once: out = alpha * Y + beta * out.
+// ComplexF64 uses FP64 MFMA; ComplexF32 uses FP32 MFMA with FP32
+// accumulation.  Columns are distributed over persistent workgroups.
+//
+// MFMA lane layouts (checked on gfx950 with a one-wave probe):
+//   A-frag  A[i][k]  lane i + 16 k          (i < 16, k < 4)
+//   B-frag  B[k][j]  lane j + 16 k
+//   f32 D   D[4 (lane/16) + r][lane % 16]   register r
+//   f64 D   D[(lane/16) + 4 r][lane % 16]   register r
+#include "qob_internal.h"
+
+namespace qob {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename R> struct mf;
+template <> struct mf<double> {
+    using v4 = v4d;
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct mf<float> {
+    using v4 = v4f;
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+constexpr int NTH = 512;   // 8 waves
+constexpr int NW = NTH / 64;
+constexpr int GPW = 2;     // fibre groups in flight per wave
+
+// Each wave owns groups of 16 fibres; for a group it computes every 16-row
+// output tile (ntm = d/16 of them) before writing the group back, so groups
+// never overlap and no barrier is needed between compute and write-back.
+// MAXNTM: d/16 of the widest factor; EPT: column elements per lane.
+template <typename R, int MAXNTM, int EPT>
+__global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *__restrict__ in,
+                                                   cplx<R> *__restrict__ out, int64_t ncols, double2 alpha_,
+                                                   double2 beta_, int beta_zero) {
+    using T = cplx<R>;
+    using V = typename mf<R>::v4;
+    extern __shared__ unsigned char lds_raw[];
+    R *re = (R *)lds_raw;
+    const int D = M.D;
+    const int Dp = D + (D >> 4);
+    R *im = re + Dp;
+    T *Al = (T *)(im + Dp + (Dp & 1));  // factor of the current mode, ld = d + 1
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
+
+    T pre[EPT];
+    int64_t col = blockIdx.x;
+    if (col < ncols) {
+        const T *src = in + col * (int64_t)D;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NTH;
+            if (e < D) pre[k] = src[e];
+        }
+    }
+    for (; col < ncols; col += gridDim.x) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NTH;
+            if (e < D) {
+                re[pad(e)] = pre[k].x;
+                im[pad(e)] = pre[k].y;
+            }
+        }
+        // issue the next column's loads now; they complete during the MFMAs
+        const int64_t nxt = col + gridDim.x;
+        if (nxt < ncols) {
+            const T *src = in + nxt * (int64_t)D;
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                const int e = tid + k * NTH;
+                if (e < D) pre[k] = src[e];
+            }
+        }
+
+        for (int s = 0; s < M.nmodes; ++s) {
+            const int n1 = M.n1[s], d = M.d[s];
+            const int lda = d + 1;
+            __syncthreads();  // previous mode written back, column loaded
+            {
+                const T *__restrict__ A = (const T *)M.A[s];
+                for (int e = tid; e < d * d; e += NTH) {
+                    const int i = e % d, k = e / d;
+                    Al[i + k * lda] = A[e];
+                }
+            }
+            __syncthreads();
+            const int ntm = d >> 4, ngroups = (D / d) >> 4;
+            const int nks = d >> 2;
+            for (int g0 = wave; g0 < ngroups; g0 += NW * GPW) {
+                V accr[GPW][MAXNTM], acci[GPW][MAXNTM];
+                int base[GPW];
+#pragma unroll
+                for (int q = 0; q < GPW; ++q) {
+                    const int g = g0 + q * NW;
+                    const int f = g * 16 + lr;
+                    const int i1 = f % n1, i3 = f / n1;
+                    base[q] = i1 + n1 * d * i3;
+#pragma unroll
+                    for (int mt = 0; mt < MAXNTM; ++mt) {
+                        accr[q][mt] = V{0, 0, 0, 0};
+                        acci[q][mt] = V{0, 0, 0, 0};
+                    }
+                }
+#pragma unroll
+                for (int ks = 0; ks < MAXNTM * 4; ++ks) {
+                    if (ks >= nks) break;
+                    R xr[GPW], xi[GPW];
+#pragma unroll
+                    for (int q = 0; q < GPW; ++q) {
+                        const int idx = (g0 + q * NW < ngroups) ? pad(base[q] + n1 * (4 * ks + lk)) : 0;
+                        xr[q] = re[idx];
+                        xi[q] = im[idx];
+                    }
+#pragma unroll
+                    for (int mt = 0; mt < MAXNTM; ++mt) {
+                        if (mt < ntm) {
+                            const T a = Al[(mt * 16 + lr) + (4 * ks + lk) * lda];
+#pragma unroll
+                            for (int q = 0; q < GPW; ++q) {
+                                accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
+                                accr[q][mt] = mf<R>::mma(a.y, -xi[q], accr[q][mt]);
+                                acci[q][mt] = mf<R>::mma(a.x, xi[q], acci[q][mt]);
+                                acci[q][mt] = mf<R>::mma(a.y, xr[q], acci[q][mt]);
+                            }
+                        }
+                    }
+                }
+                // the group's inputs are no longer needed by anyone: write back
+#pragma unroll
+                for (int q = 0; q < GPW; ++q) {
+                    if (g0 + q * NW < ngroups) {
+#pragma unroll
+                        for (int mt = 0; mt < MAXNTM; ++mt) {
+                            if (mt < ntm) {
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    const int a = mt * 16 + mf<R>::row(lane, r);
+                                    const int idx = pad(base[q] + n1 * a);
+                                    re[idx] = accr[q][mt][r];
+                                    im[idx] = acci[q][mt][r];
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        T *dst = out + col * (int64_t)D;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NTH;
+            if (e < D) {
+                const T y = mk(re[pad(e)], im[pad(e)]);
+                T r = cmul(alpha, y);
+                if (!beta_zero) r = cfma(beta, dst[e], r);
+                st_stream(dst + e, r);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// LDS bytes for a column of D elements (padded planes)
+static size_t fused_lds(qob_dtype dt, int D, int dmax) {
+    const size_t r = (dt == QOB_C64 ? 8 : 4);
+    const size_t dp = (size_t)(D + (D >> 4));
+    return (2 * dp + (dp & 1)) * r + (size_t)dmax * (dmax + 1) * 2 * r;
+}
+
+static int fused_dmax(const FusedModes &M) {
+    int m = 0;
+    for (int i = 0; i < M.nmodes; ++i) m = std::max(m, M.d[i]);
+    return m;
+}
+
+bool fused_supported(qob_dtype dt, const FusedModes &M) {
+    if (M.nmodes < 1 || M.nmodes > FUSED_MAXMODES || M.D <= 0 || (M.D & 255)) return false;
+    const int dmax = (dt == QOB_C64) ? 8192 : 16384;
+    if (M.D > dmax) return false;
+    if (fused_lds(dt, M.D, fused_dmax(M)) > 160 * 1024) return false;
+    int64_t prodd = 1;
+    for (int s = 0; s < M.nmodes; ++s) {
+        const int d = M.d[s];
+        if (d != 16 && d != 32 && d != 64) return false;  // d/16 divides the 8 waves
+        if (((M.D / d) & 15) != 0) return false;
+        if ((int64_t)M.n1[s] * d > M.D) return false;
+        prodd *= d;
+    }
+    return prodd <= M.D;
+}
+
+hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols,
+                           double2 alpha, double2 beta, hipStream_t s) {
+    if (ncols == 0) return hipSuccess;
+    if (!fused_supported(dt, M)) return hipErrorInvalidValue;
+    const int bz = (beta.x == 0.0 && beta.y == 0.0);
+    const int maxd = fused_dmax(M);
+    const size_t lds = fused_lds(dt, M.D, maxd);
+    const int ept = (M.D + NTH - 1) / NTH;
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
+    }
+    const int64_t per_cu = std::max<int64_t>(1, (160 * 1024) / (int64_t)lds);
+    const unsigned g = (unsigned)std::min<int64_t>(ncols, (int64_t)ncu * per_cu);
+#define QOB_FU(RR, NTM, EP)                                                                                  \
+    do {                                                                                                     \
+        auto kfn = k_lt_fused<RR, NTM, EP>;                                                                  \
+        hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                            (int)lds);                                                       \
+        if (e_ != hipSuccess) return e_;                                                                     \
+        hipLaunchKernelGGL(kfn, dim3(g), dim3(NTH), lds, s, M, (const cplx<RR> *)in, (cplx<RR> *)out, ncols, \
+                           alpha, beta, bz);                                                                 \
+        return hipGetLastError();                                                                            \
+    } while (0)
+    const int ntm = maxd / 16;
+    if (dt == QOB_C64) {
+        if (ept <= 4) { if (ntm <= 2) QOB_FU(double, 2, 4); else QOB_FU(double, 4, 4); }
+        if (ept <= 8) { if (ntm <= 2) QOB_FU(double, 2, 8); else QOB_FU(double, 4, 8); }
+        if (ept <= 16) { if (ntm <= 2) QOB_FU(double, 2, 16); else QOB_FU(double, 4, 16); }
+    } else {
+        if (ept <= 8) { if (ntm <= 2) QOB_FU(float, 2, 8); else QOB_FU(float, 4, 8); }
+        if (ept <= 16) { if (ntm <= 2) QOB_FU(float, 2, 16); else QOB_FU(float, 4, 16); }
+        if (ept <= 32) { if (ntm <= 2) QOB_FU(float, 2, 32); else QOB_FU(float, 4, 32); }
+    }
+    return hipErrorInvalidValue;
+#undef QOB_FU
+    return hipGetLastError();
+}
+
+}  // namespace qob

@@ -1,0 +1,480 @@
+// Fused LazySum of qubit Pauli-type terms, for one state or a batch of
+// states of different lengths in one persistent launch.
+//
+// Reference: mul!(::Ket, ::LazySum, ::Ket) (src/operators_lazysum.jl:189-200)
+// runs each LazyTensor term through the recursive scatter
+// _gemm_recursive_lazy_dense (src/operators_lazytensor.jl:652-685), one full
+// read-modify-write of the result per term.  When every site is a qubit and
+// every factor is monomial (at most one entry per row: sigma_x/y/z, sigma_+-,
+// number, projectors), a term maps basis state J to J ^ F_t with weight
+// c_t * prod_s v_s[bit_s(J)].  The host groups the terms by flip mask F and
+// folds each group's weight into either a constant or a sum of small tables
+// indexed by a window of <= 8 bits of J:
+//   out[J] = alpha * sum_F w_F(J) * in[J ^ F] + gamma * in[J] + beta * out[J]
+// gamma fuses the Ket update of a time step (psi + c H psi) into the apply.
+//
+// A workgroup owns a tile of 2^tb consecutive basis states: the tile is read
+// once into LDS, flips inside the tile read LDS, flips of higher bits read
+// global memory (coalesced, served by L2 / Infinity Cache).  Work items
+// (state, tile) of all states of a batch share one launch.
+//
+// Program of a state (int32, wave-uniform reads):
+//   [ngroups] then per group [F, nwin, const_off] followed by nwin windows
+//   [shift, nbits, table_off]; nwin == 0 means the weight is vals[const_off].
+#include "qob_internal.h"
+
+namespace qob {
+
+namespace {
+
+// logical workgroup id such that each of the 8 XCDs owns a contiguous range
+// (linear id b runs on XCD b % 8): tiles of one state that differ in high
+// bits are then processed on the same XCD and share its L2.
+__device__ __forceinline__ uint32_t qb_xcd_remap(uint32_t b, uint32_t G) {
+    const uint32_t x = b & 7u, i = b >> 3;
+    const uint32_t q = G >> 3, r = G & 7u;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+template <typename T> __device__ __forceinline__ T to_t(double2 a) {
+    T r; r.x = (typename real_of<T>::type)a.x; r.y = (typename real_of<T>::type)a.y; return r;
+}
+
+// acc += w * x for every element of the lane, w uniform; a real weight
+// needs two FMAs instead of four.  Partner of element k (tile index
+// e = tid + k NTH) under flip F is e ^ F = (tid ^ F) ^ (k NTH): one XOR with a
+// compile-time constant per element.
+template <typename T, int NTH, int EPT, bool CHECK>
+__device__ __forceinline__ void qb_uniform(T wt, const char *basep, uint32_t tpb, int tsize, T (&acc)[EPT]) {
+    const int tid = threadIdx.x;
+    if (wt.y == 0) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            if (CHECK && tid + k * NTH >= tsize) continue;
+            const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+            acc[k].x = fma(wt.x, x.x, acc[k].x);
+            acc[k].y = fma(wt.x, x.y, acc[k].y);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            if (CHECK && tid + k * NTH >= tsize) continue;
+            const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+            acc[k] = cfma(wt, x, acc[k]);
+        }
+    }
+}
+
+template <typename T, int NTH, int EPT, bool CHECK>
+__device__ __forceinline__ void qb_groups(const int32_t *__restrict__ pg, const T *__restrict__ vals, const T *tile,
+                                          const T *__restrict__ in, uint32_t base, int tb, int tsize,
+                                          T (&acc)[EPT]) {
+    const int tid = threadIdx.x;
+    const int ngroups = pg[0];
+    pg += 1;
+    const uint32_t lowmask = (uint32_t)tsize - 1u;
+    for (int g = 0; g < ngroups; ++g) {
+        const uint32_t F = (uint32_t)pg[0];
+        const int nwin = pg[1];
+        const int coff = pg[2];
+        pg += 3;
+        const bool in_tile = (F >> tb) == 0;
+        // partner addressing: LDS tile, or the partner tile in global memory
+        const uint32_t tpb = ((uint32_t)tid ^ (F & lowmask)) * (uint32_t)sizeof(T);
+        if (nwin == 0) {
+            const T wt = vals[coff];
+            if (wt.x == 0 && wt.y == 0) continue;
+            if (in_tile) {
+                qb_uniform<T, NTH, EPT, CHECK>(wt, (const char *)tile, tpb, tsize, acc);
+            } else {
+                const char *gp = (const char *)(in + (base ^ (F & ~lowmask)));
+                qb_uniform<T, NTH, EPT, CHECK>(wt, gp, tpb, tsize, acc);
+            }
+        } else {
+            const char *basep = (const char *)(in + (base ^ (F & ~lowmask)));
+            T wt[EPT];
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) wt[k] = czero<T>();
+            for (int q = 0; q < nwin; ++q) {
+                const int sh = pg[0], nb = pg[1], toff = pg[2];
+                pg += 3;
+                const uint32_t mask = (1u << nb) - 1u;
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    const uint32_t J = base + (uint32_t)(tid + k * NTH);
+                    wt[k] = cadd(wt[k], vals[toff + (int)((J >> sh) & mask)]);
+                }
+            }
+            if (in_tile) {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    if (CHECK && tid + k * NTH >= tsize) continue;
+                    if (wt[k].x != 0 || wt[k].y != 0) {
+                        const T x = *(const T *)((const char *)tile + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+                        acc[k] = cfma(wt[k], x, acc[k]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    if (CHECK && tid + k * NTH >= tsize) continue;
+                    if (wt[k].x != 0 || wt[k].y != 0) {
+                        const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+                        acc[k] = cfma(wt[k], x, acc[k]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// TBITS: log2 of the tile; each lane owns EPT = 2^TBITS / NTH elements of a
+// tile (e = tid + k * NTH) and the group loop is outermost, so the program is
+// decoded once per group and the lane's EPT partner loads are independent.
+template <typename T, int NTH, int TBITS>
+__global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__ states,
+                                               const QubitItemDev *__restrict__ items, int nitems,
+                                               const int32_t *__restrict__ prog, const T *__restrict__ vals,
+                                               double2 alpha_, double2 beta_, double2 gamma_, int flags,
+                                               QubitStateDev single, int ntiles1) {
+    constexpr int EPT = (1 << TBITS) / NTH;
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = (flags & 4) ? qb_xcd_remap(blockIdx.x, G) : blockIdx.x;
+    extern __shared__ unsigned char lds_raw[];
+    T *tile = (T *)lds_raw;
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_), gamma = to_t<T>(gamma_);
+    const bool beta_zero = flags & 1, gamma_zero = flags & 2;
+    const int tid = threadIdx.x;
+    // items in rounds of G: all workgroups work on neighbouring tiles at the
+    // same time (partners of high-bit flips are being read concurrently), and
+    // within a round each XCD owns a contiguous run of tiles
+    for (int it = (int)lb; it < nitems; it += (int)G) {
+        QubitItemDev w;
+        QubitStateDev S;
+        if (states) {
+            w = items[it];
+            S = states[w.state];
+        } else {
+            // columns form: state c is column c of a 2^nbits x ncols matrix
+            const int c = it / ntiles1;
+            w.state = c;
+            w.tile = it - c * ntiles1;
+            S = single;
+            S.in = (const T *)single.in + ((int64_t)c << single.nbits);
+            S.out = (T *)single.out + ((int64_t)c << single.nbits);
+        }
+        const int tb = S.nbits < TBITS ? S.nbits : TBITS;
+        const int tsize = 1 << tb;
+        const uint32_t base = (uint32_t)w.tile << tb;
+        const T *__restrict__ in = (const T *)S.in;
+        T *__restrict__ out = (T *)S.out;
+        // a second pass already wrote alpha * high + beta * out: add to it
+        const bool add_out = S.prog_hi_off >= 0;
+        const bool bz = beta_zero && !add_out;
+        const T beff = add_out ? mk((typename real_of<T>::type)1, (typename real_of<T>::type)0) : beta;
+        __syncthreads();  // the previous item's tile reads are finished
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NTH;
+            if (e < tsize) tile[e] = in[base + e];
+        }
+        __syncthreads();
+        T acc[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
+        if (tsize == (1 << TBITS))
+            qb_groups<T, NTH, EPT, false>(prog + S.prog_off, vals, tile, in, base, tb, tsize, acc);
+        else
+            qb_groups<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, in, base, tb, tsize, acc);
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NTH;
+            if (e < tsize) {
+                const uint32_t J = base + (uint32_t)e;
+                T r = cmul(alpha, acc[k]);
+                if (!gamma_zero) r = cfma(gamma, tile[e], r);
+                if (!bz) r = cfma(beff, out[J], r);
+                st_stream(out + J, r);
+            }
+        }
+    }
+}
+
+// Second pass: flips of bits above the tile.  With h = nbits - TBITS high
+// bits and c = TBITS - h low bits, tile m holds the 2^TBITS basis states
+//   J(q) = (q >> c) << TBITS | m << c | (q & (2^c - 1)),
+// i.e. every combination of the high bits times a contiguous run of 2^c
+// states, so a flip of high bits (and of bits below c) stays inside the
+// tile.  Writes out = alpha * sum_high + beta * out.
+template <typename T, int NTH, int TBITS>
+__global__ void __launch_bounds__(NTH) k_qubit_hi(const QubitStateDev *__restrict__ states,
+                                                  const QubitItemDev *__restrict__ items, int nitems,
+                                                  const int32_t *__restrict__ prog, const T *__restrict__ vals,
+                                                  double2 alpha_, double2 beta_, int flags, QubitStateDev single,
+                                                  int ntiles1) {
+    constexpr int EPT = (1 << TBITS) / NTH;
+    extern __shared__ unsigned char lds_raw[];
+    T *tile = (T *)lds_raw;
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+    const bool beta_zero = flags & 1;
+    const int tid = threadIdx.x;
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = (flags & 4) ? qb_xcd_remap(blockIdx.x, G) : blockIdx.x;
+    for (int it = (int)lb; it < nitems; it += (int)G) {
+        QubitItemDev w;
+        QubitStateDev S;
+        if (states) {
+            w = items[it];
+            S = states[w.state];
+        } else {
+            const int cidx = it / ntiles1;
+            w.state = cidx;
+            w.tile = it - cidx * ntiles1;
+            S = single;
+            S.in = (const T *)single.in + ((int64_t)cidx << single.nbits);
+            S.out = (T *)single.out + ((int64_t)cidx << single.nbits);
+        }
+        const int h = S.nbits - TBITS;
+        const int c = TBITS - h;
+        const uint32_t lowmask = (1u << c) - 1u;
+        const uint32_t mid = (uint32_t)w.tile << c;
+        const T *__restrict__ in = (const T *)S.in;
+        T *__restrict__ out = (T *)S.out;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const uint32_t q = (uint32_t)(tid + k * NTH);
+            tile[q] = in[((q >> c) << TBITS) | mid | (q & lowmask)];
+        }
+        __syncthreads();
+        T acc[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
+        const int32_t *pg = prog + S.prog_hi_off;
+        const int ngroups = pg[0];
+        pg += 1;
+        for (int g = 0; g < ngroups; ++g) {
+            const uint32_t F = (uint32_t)pg[0];
+            const int nwin = pg[1];
+            const int coff = pg[2];
+            pg += 3;
+            const uint32_t Flo = F & ((1u << TBITS) - 1u);
+            const bool in_tile = (Flo >> c) == 0;
+            const uint32_t Fq = ((F >> TBITS) << c) | Flo;  // the flip in tile coordinates
+            T wt[EPT];
+            if (nwin == 0) {
+                const T u = vals[coff];
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) wt[k] = u;
+            } else {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) wt[k] = czero<T>();
+                for (int qq = 0; qq < nwin; ++qq) {
+                    const int sh = pg[0], nb = pg[1], toff = pg[2];
+                    pg += 3;
+                    const uint32_t mask = (1u << nb) - 1u;
+#pragma unroll
+                    for (int k = 0; k < EPT; ++k) {
+                        const uint32_t q = (uint32_t)(tid + k * NTH);
+                        const uint32_t J = ((q >> c) << TBITS) | mid | (q & lowmask);
+                        wt[k] = cadd(wt[k], vals[toff + (int)((J >> sh) & mask)]);
+                    }
+                }
+            }
+            if (in_tile) {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    const uint32_t q = (uint32_t)(tid + k * NTH);
+                    if (wt[k].x != 0 || wt[k].y != 0) acc[k] = cfma(wt[k], tile[q ^ Fq], acc[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    const uint32_t q = (uint32_t)(tid + k * NTH);
+                    const uint32_t J = ((q >> c) << TBITS) | mid | (q & lowmask);
+                    if (wt[k].x != 0 || wt[k].y != 0) acc[k] = cfma(wt[k], in[J ^ F], acc[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const uint32_t q = (uint32_t)(tid + k * NTH);
+            const uint32_t J = ((q >> c) << TBITS) | mid | (q & lowmask);
+            T r = cmul(alpha, acc[k]);
+            if (!beta_zero) r = cfma(beta, out[J], r);
+            out[J] = r;
+        }
+    }
+}
+
+// Elementwise batch update out += a * in over the same items (the axpy of a
+// time step for every state of a batch).
+template <typename T, int NTH>
+__global__ void __launch_bounds__(NTH) k_qubit_axpy(const QubitStateDev *__restrict__ states,
+                                                    const QubitItemDev *__restrict__ items, int nitems, int tbits,
+                                                    double2 a_) {
+    const T a = to_t<T>(a_);
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const QubitItemDev w = items[it];
+        const QubitStateDev S = states[w.state];
+        const int tb = S.nbits < tbits ? S.nbits : tbits;
+        const uint32_t base = (uint32_t)w.tile << tb;
+        const T *__restrict__ in = (const T *)S.in;
+        T *__restrict__ out = (T *)S.out;
+        for (int e = threadIdx.x; e < (1 << tb); e += NTH) out[base + e] = cfma(a, in[base + e], out[base + e]);
+    }
+}
+
+}  // namespace
+
+int g_qubit_nth = 256;
+int g_qubit_remap = 1;
+int g_qubit_grid = 0;  // 0: 4 workgroups per CU
+
+static unsigned qubit_grid(int nitems) {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
+    }
+    const int64_t g = g_qubit_grid > 0 ? g_qubit_grid : (int64_t)ncu * 4;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, nitems));
+}
+
+int g_qubit_tbits = 12;
+
+static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
+                                   int tbits, const int32_t *prog, const void *vals, double2 alpha, double2 beta,
+                                   double2 gamma, hipStream_t s, QubitStateDev single, int ntiles1);
+
+hipError_t launch_qubit(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
+                        const int32_t *prog, const void *vals, double2 alpha, double2 beta, double2 gamma,
+                        hipStream_t s) {
+    return launch_qubit_any(dt, states, items, nitems, tbits, prog, vals, alpha, beta, gamma, s, QubitStateDev(), 1);
+}
+
+hipError_t launch_qubit_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
+                             const void *vals, double2 alpha, double2 beta, double2 gamma, hipStream_t s) {
+    const int tb = std::min(tbits, (int)single.nbits);
+    const int64_t ntiles = (int64_t)1 << (single.nbits - tb);
+    const int64_t nitems = ntiles * ncols;
+    if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
+    return launch_qubit_any(dt, nullptr, nullptr, (int)nitems, tbits, prog, vals, alpha, beta, gamma, s, single,
+                            (int)ntiles);
+}
+
+static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
+                                   int tbits, const int32_t *prog, const void *vals, double2 alpha, double2 beta,
+                                   double2 gamma, hipStream_t s, QubitStateDev single, int ntiles1) {
+    if (nitems == 0) return hipSuccess;
+    const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (gamma.x == 0.0 && gamma.y == 0.0 ? 2 : 0) |
+                      (g_qubit_remap ? 4 : 0);
+    if (tbits < 10) tbits = 10;
+    if (tbits > 13) tbits = 13;
+    const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const unsigned g = qubit_grid(nitems);
+#define QOB_QB(TT, NT, TB)                                                                                    \
+    do {                                                                                                      \
+        auto kfn = k_qubit<TT, NT, TB>;                                                                       \
+        static size_t lds_set = 0; /* raise the limit once, outside any stream capture */                     \
+        if (lds > lds_set) {                                                                                  \
+            hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                (int)(160 * 1024));                                           \
+            if (e_ != hipSuccess) return e_;                                                                  \
+            lds_set = 160 * 1024;                                                                             \
+        }                                                                                                     \
+        hipLaunchKernelGGL(kfn, dim3(g), dim3(NT), lds, s, states, items, nitems, prog, (const TT *)vals,     \
+                           alpha, beta, gamma, flags, single, ntiles1);                                      \
+        return hipGetLastError();                                                                             \
+    } while (0)
+#define QOB_QB_NT(TT, TB)                                  \
+    do {                                                   \
+        if (g_qubit_nth == 512) QOB_QB(TT, 512, TB);       \
+        else if (g_qubit_nth == 1024) QOB_QB(TT, 1024, TB); \
+        else QOB_QB(TT, 256, TB);                          \
+    } while (0)
+#define QOB_QB_TB(TT)                          \
+    do {                                       \
+        if (tbits <= 10) QOB_QB_NT(TT, 10);    \
+        else if (tbits == 11) QOB_QB_NT(TT, 11); \
+        else if (tbits == 12) QOB_QB_NT(TT, 12); \
+        else QOB_QB_NT(TT, 13);                \
+    } while (0)
+    if (dt == QOB_C64) QOB_QB_TB(double2);
+    else QOB_QB_TB(float2);
+#undef QOB_QB_TB
+#undef QOB_QB_NT
+#undef QOB_QB
+    return hipGetLastError();
+}
+
+static hipError_t launch_qubit_hi_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items,
+                                      int nitems, int tbits, const int32_t *prog, const void *vals, double2 alpha,
+                                      double2 beta, hipStream_t s, QubitStateDev single, int ntiles1) {
+    if (nitems == 0) return hipSuccess;
+    if (tbits < 10 || tbits > 13) return hipErrorInvalidValue;
+    const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (g_qubit_remap ? 4 : 0);
+    const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
+    const unsigned g = qubit_grid(nitems);
+#define QOB_QH(TT, NT, TB)                                                                                    \
+    do {                                                                                                      \
+        auto kfn = k_qubit_hi<TT, NT, TB>;                                                                    \
+        static size_t lds_set = 0;                                                                            \
+        if (lds > lds_set) {                                                                                  \
+            hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                (int)(160 * 1024));                                           \
+            if (e_ != hipSuccess) return e_;                                                                  \
+            lds_set = 160 * 1024;                                                                             \
+        }                                                                                                     \
+        hipLaunchKernelGGL(kfn, dim3(g), dim3(NT), lds, s, states, items, nitems, prog, (const TT *)vals,     \
+                           alpha, beta, flags, single, ntiles1);                                              \
+        return hipGetLastError();                                                                             \
+    } while (0)
+#define QOB_QH_NT(TT, TB)                                  \
+    do {                                                   \
+        if (g_qubit_nth == 512) QOB_QH(TT, 512, TB);       \
+        else if (g_qubit_nth == 1024) QOB_QH(TT, 1024, TB); \
+        else QOB_QH(TT, 256, TB);                          \
+    } while (0)
+#define QOB_QH_TB(TT)                            \
+    do {                                         \
+        if (tbits == 10) QOB_QH_NT(TT, 10);      \
+        else if (tbits == 11) QOB_QH_NT(TT, 11); \
+        else if (tbits == 12) QOB_QH_NT(TT, 12); \
+        else QOB_QH_NT(TT, 13);                  \
+    } while (0)
+    if (dt == QOB_C64) QOB_QH_TB(double2);
+    else QOB_QH_TB(float2);
+#undef QOB_QH_TB
+#undef QOB_QH_NT
+#undef QOB_QH
+}
+
+hipError_t launch_qubit_hi(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
+                           const int32_t *prog, const void *vals, double2 alpha, double2 beta, hipStream_t s) {
+    return launch_qubit_hi_any(dt, states, items, nitems, tbits, prog, vals, alpha, beta, s, QubitStateDev(), 1);
+}
+
+hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
+                                const void *vals, double2 alpha, double2 beta, hipStream_t s) {
+    if (single.nbits <= tbits || single.prog_hi_off < 0) return hipSuccess;
+    const int64_t ntiles = (int64_t)1 << (single.nbits - tbits);
+    const int64_t nitems = ntiles * ncols;
+    if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
+    return launch_qubit_hi_any(dt, nullptr, nullptr, (int)nitems, tbits, prog, vals, alpha, beta, s, single,
+                               (int)ntiles);
+}
+
+hipError_t launch_qubit_axpy(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
+                             int tbits, double2 a, hipStream_t s) {
+    if (nitems == 0) return hipSuccess;
+    const unsigned g = qubit_grid(nitems);
+    if (dt == QOB_C64)
+        hipLaunchKernelGGL((k_qubit_axpy<double2, 256>), dim3(g), dim3(256), 0, s, states, items, nitems, tbits, a);
+    else
+        hipLaunchKernelGGL((k_qubit_axpy<float2, 256>), dim3(g), dim3(256), 0, s, states, items, nitems, tbits, a);
+    return hipGetLastError();
+}
+
+}  // namespace qob

@@ -190,4 +190,49 @@ hipError_t launch_modek(qob_dtype dt, const void *A, int64_t d_out, int64_t d_in
                         hipStream_t s);
 bool modek_supported(int64_t d_out, int64_t d_in);
 
+// fused dense LazyTensor on a column (k_fused.hip): modes applied in order,
+// mode s acts on dimension d[s] with n1[s] = product of the dims before it;
+// A[s] is the d x d factor (column-major, dtype of the launch).
+enum { FUSED_MAXMODES = 8 };
+struct FusedModes {
+    int nmodes = 0;
+    int D = 0;
+    int n1[FUSED_MAXMODES] = {0};
+    int d[FUSED_MAXMODES] = {0};
+    const void *A[FUSED_MAXMODES] = {nullptr};
+};
+bool fused_supported(qob_dtype dt, const FusedModes &M);
+
+// qubit Pauli-type LazySum (k_qubit.hip)
+struct QubitStateDev {
+    int32_t nbits = 0;          // state length 2^nbits
+    int32_t prog_off = 0;       // program of the flips inside a tile (all flips if prog_hi_off < 0)
+    int32_t prog_hi_off = -1;   // program of the flips of bits above the tile (second pass), or -1
+    int32_t pad_ = 0;
+    const void *in = nullptr;
+    void *out = nullptr;
+};
+struct QubitItemDev {
+    int32_t state;
+    int32_t tile;
+};
+// Batched form: items (state, tile) index a device table of states.
+hipError_t launch_qubit(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
+                        const int32_t *prog, const void *vals, double2 alpha, double2 beta, double2 gamma,
+                        hipStream_t s);
+// Columns form: ncols states of one program, state c at in/out + c * 2^nbits.
+hipError_t launch_qubit_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
+                             const void *vals, double2 alpha, double2 beta, double2 gamma, hipStream_t s);
+hipError_t launch_qubit_axpy(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
+                             int tbits, double2 a, hipStream_t s);
+// Second pass over strided tiles (high-bit flips) for states with prog_hi_off >= 0:
+// out = alpha * high + beta * out; the first pass then adds to out.
+hipError_t launch_qubit_hi(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
+                           const int32_t *prog, const void *vals, double2 alpha, double2 beta, hipStream_t s);
+hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
+                                const void *vals, double2 alpha, double2 beta, hipStream_t s);
+// out (D x ncols) = alpha * (A_n ... A_1) in + beta * out
+hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols,
+                           double2 alpha, double2 beta, hipStream_t s);
+
 }  // namespace qob

@@ -11,5 +11,6 @@ from .core import (AbstractOperator, Basis, Bra, CompositeBasis, Context, DenseO
                    synchronize, tensor, tr)
 from .spin import sigmam, sigmap, sigmax, sigmay, sigmaz
 from .fock import create, destroy, number
+from .batch import Batch, Graph
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -108,6 +108,16 @@ _PROTOS = {
     "qob_normalize_op": (_i32, [_vp, _i32, _i64, _vp, _vp]),
     "qob_expect_ket": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
     "qob_expect_op": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
+    "qob_batch_create": (_i32, [_vp, _i32, _P(_vp), _i32, _P(_vp)]),
+    "qob_batch_bind": (_i32, [_vp, _vp, _i32, _P(_vp), _P(_vp)]),
+    "qob_batch_apply": (_i32, [_vp, _vp, _i32, qob_c64, qob_c64, qob_c64, _vp]),
+    "qob_batch_axpy": (_i32, [_vp, _vp, _i32, qob_c64, _vp]),
+    "qob_batch_info": (_i32, [_vp, _P(_i64), _P(_i32)]),
+    "qob_batch_destroy": (_i32, [_vp]),
+    "qob_graph_begin": (_i32, [_vp, _vp]),
+    "qob_graph_end": (_i32, [_vp, _vp, _P(_vp)]),
+    "qob_graph_launch": (_i32, [_vp, _vp, _vp]),
+    "qob_graph_destroy": (_i32, [_vp]),
     "qob_dot_d": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp]),
     "qob_norm_d": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp]),
     "qob_trace_d": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp]),

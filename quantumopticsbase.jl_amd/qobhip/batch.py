@@ -1,0 +1,86 @@
+"""Batched application of many qubit LazySums in one launch, and HIP graph
+capture of library call sequences (C entry points qob_batch_* and
+qob_graph_*).  No reference counterpart: the reference applies one operator
+per mul! call; these exist for time stepping many states at once.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import c64, call
+from .core import DeviceArray, Ket, _dt, context
+
+
+class Batch:
+    """ops[j] acts on states of length 2^L_j; one launch applies all of them."""
+
+    def __init__(self, ops: Sequence, dtype=np.complex128):
+        self.ops = list(ops)
+        self.dtype = np.dtype(dtype)
+        ctx = context()
+        hs = (C.c_void_p * len(ops))(*[op._handle().value for op in ops])
+        h = C.c_void_p()
+        call("qob_batch_create", ctx.h, len(ops), hs, _dt(self.dtype), C.byref(h))
+        self.h = h
+        self._bufs = {}
+
+    def info(self):
+        tot, n = C.c_int64(), C.c_int()
+        call("qob_batch_info", self.h, C.byref(tot), C.byref(n))
+        return tot.value, n.value
+
+    def bind(self, slot: int, ins: Sequence[DeviceArray], outs: Sequence[DeviceArray]) -> None:
+        """Bind per-state input/output device buffers to a slot."""
+        pin = (C.c_void_p * len(ins))(*[d.ptr.value for d in ins])
+        pout = (C.c_void_p * len(outs))(*[d.ptr.value for d in outs])
+        call("qob_batch_bind", context().h, self.h, slot, pin, pout)
+        self._bufs[slot] = (list(ins), list(outs))
+
+    def apply(self, slot: int, alpha=1.0, beta=0.0, gamma=0.0, stream=None) -> None:
+        """out_j = alpha op_j in_j + gamma in_j + beta out_j for every j."""
+        call("qob_batch_apply", context().h, self.h, slot, c64(alpha), c64(beta), c64(gamma), stream)
+
+    def axpy(self, slot: int, a, stream=None) -> None:
+        """out_j += a in_j for every j."""
+        call("qob_batch_axpy", context().h, self.h, slot, c64(a), stream)
+
+    def __del__(self):
+        try:
+            if self.h and self.h.value:
+                _lib.load().qob_batch_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Graph:
+    """Capture library calls issued inside ``with Graph() as g:`` into a HIP
+    graph; ``g.launch()`` replays them.  Every captured call must have run
+    once before capture."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+        self.h = None
+
+    def __enter__(self):
+        call("qob_graph_begin", context().h, self.stream)
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = C.c_void_p()
+        call("qob_graph_end", context().h, self.stream, C.byref(h))
+        self.h = h
+        return False
+
+    def launch(self) -> None:
+        call("qob_graph_launch", context().h, self.h, self.stream)
+
+    def __del__(self):
+        try:
+            if self.h is not None and self.h.value:
+                _lib.load().qob_graph_destroy(self.h)
+        except Exception:
+            pass

@@ -283,3 +283,49 @@ def test_expect_matches_oracle():
     want = O.expect_op(lt, rho)
     got = Q.expect(q, Q.Operator(q.basis_r, q.basis_r, rho))
     assert abs(got - want) < 1e-12 * abs(want)
+
+
+# ------------------------------------------------ fused dense LazyTensor
+FUSED_CASES = [
+    ((16,), (1,)),
+    ((16, 32), (1, 2)),
+    ((32, 16), (1, 2)),
+    ((16, 2, 16), (1, 3)),      # identity site between factors
+    ((2, 16, 16), (2, 3)),      # identity fastest site: n1 = 2
+    ((16, 32, 16), (1, 2, 3)),
+    ((64, 16), (1, 2)),
+]
+
+
+def _tune(name, v):
+    from qobhip import _lib
+    _lib.call("qob_tune_set", name.encode(), v)
+
+
+@pytest.mark.parametrize("case", range(len(FUSED_CASES)))
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_fused_dense_lazytensor(case, dtype, tol):
+    shape, idx = FUSED_CASES[case]
+    g = np.random.default_rng(70 + case)
+    lt = rand_lt(g, shape, shape, idx, ("dense",) * len(idx))
+    q = to_q(lt)
+    run_all_paths(lt, q, g, 0.8 - 0.3j, 0.4 + 0.1j, dtype, tol, ncols=5)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_fused_equals_chain(dtype):
+    g = np.random.default_rng(80)
+    lt = rand_lt(g, (16, 32, 16), (16, 32, 16), (1, 2, 3), ("dense",) * 3)
+    q = to_q(lt)
+    B = cg(g, 8192, 3)
+    outs = []
+    for fused in (1, 0):
+        _tune("fused_lt", fused)
+        try:
+            R = Q.Operator(q.basis_l, gb(3), np.zeros((8192, 3)), dtype)
+            Q.mul_(R, q, Q.Operator(q.basis_r, gb(3), B, dtype), 1.0, 0.0)
+            outs.append(R.to_numpy().astype(np.complex128))
+        finally:
+            _tune("fused_lt", 1)
+    tol = TOL64 if dtype == np.complex128 else TOL32
+    assert rel(outs[0], outs[1]) < tol
