@@ -694,6 +694,8 @@ static qob_status lazy_plan(OpNode &owner, const std::string &key, const std::ve
 // group's weight w_F(J) = sum_t c_t prod_s v_{t,s}[bit_s(J)] is stored as a
 // constant or as tables over windows of <= 8 bits (program layout in
 // k_qubit.hip).  Returns false when a term does not fit this form.
+int g_qubit_split = 0;  // 1: high-bit flips in a second, strided pass; 0: partner tiles read in one pass
+
 struct QubitProgHost {
     std::vector<int32_t> prog;     // flips inside a tile (all flips when prog_hi is empty)
     std::vector<int32_t> prog_hi;  // flips of bits >= tb, for the strided second pass
@@ -743,24 +745,21 @@ static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitPr
     }
     std::map<uint32_t, std::vector<const QT *>> groups;
     for (const QT &q : qts) groups[q.F].push_back(&q);
-    Q.prog.clear();
-    Q.prog_hi.clear();
-    Q.vals.clear();
-    const bool split = nbits > tb && nbits <= 2 * tb;
-    int nlo = 0, nhi = 0;
-    for (auto &gq : groups) ((split && (gq.first >> tb) != 0) ? nhi : nlo)++;
-    Q.prog.push_back(nlo);
-    if (nhi) Q.prog_hi.push_back(nhi);
-    for (auto &gq : groups) {
-        std::vector<int32_t> &P = (split && (gq.first >> tb) != 0) ? Q.prog_hi : Q.prog;
-        // windows of width <= 8 starting at multiples of 6 where possible
+    struct GroupOut {
+        uint32_t F;
+        cd c;
         std::map<int, std::vector<cd>> win;
-        cd constant(0, 0);
-        bool has_const = false;
+    };
+    std::vector<GroupOut> outs[2];  // [0]: first pass, [1]: high-bit pass
+    const bool split = g_qubit_split && nbits > tb && nbits <= 2 * tb;
+    for (auto &gq : groups) {
+        GroupOut go;
+        go.F = gq.first;
+        go.c = cd(0, 0);
+        // windows of width <= 8 starting at multiples of 6 where possible
         for (const QT *q : gq.second) {
             if (q->sites.empty()) {
-                constant += q->coef;
-                has_const = true;
+                go.c += q->coef;
                 continue;
             }
             const int m = *std::min_element(q->sites.begin(), q->sites.end());
@@ -769,7 +768,7 @@ static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitPr
             int ws = (m / 6) * 6;
             if (M >= ws + 8) ws = m;
             const int nb = std::min(8, nbits - ws);
-            auto &tab = win[ws];
+            auto &tab = go.win[ws];
             if (tab.empty()) tab.assign((size_t)1 << nb, cd(0, 0));
             for (int idx = 0; idx < (1 << nb); ++idx) {
                 cd w = q->coef;
@@ -778,34 +777,45 @@ static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitPr
             }
         }
         // fold windows whose table is constant into the constant part
-        for (auto it = win.begin(); it != win.end();) {
+        for (auto it = go.win.begin(); it != go.win.end();) {
             bool c = true;
             for (auto &z : it->second) c = c && (z == it->second[0]);
             if (c) {
-                constant += it->second[0];
-                has_const = true;
-                it = win.erase(it);
+                go.c += it->second[0];
+                it = go.win.erase(it);
             } else {
                 ++it;
             }
         }
-        P.push_back((int32_t)gq.first);
-        if (win.empty()) {
-            P.push_back(0);
-            P.push_back((int32_t)Q.vals.size());
-            Q.vals.push_back(constant);
-            continue;
-        }
-        P.push_back((int32_t)win.size());
-        P.push_back(0);
-        bool first = true;
-        for (auto &w : win) {
-            const int nb = std::min(8, nbits - w.first);
-            P.push_back(w.first);
-            P.push_back(nb);
-            P.push_back((int32_t)Q.vals.size());
-            for (auto &z : w.second) Q.vals.push_back(first && has_const ? z + constant : z);
-            first = false;
+        if (go.win.empty() && go.c == cd(0, 0)) continue;  // the group vanishes
+        outs[(split && (go.F >> tb) != 0) ? 1 : 0].push_back(std::move(go));
+    }
+    auto put_cd = [](std::vector<int32_t> &P, cd z) {
+        int32_t w[4];
+        const double d[2] = {z.real(), z.imag()};
+        std::memcpy(w, d, sizeof(w));
+        P.insert(P.end(), w, w + 4);
+    };
+    Q.prog.clear();
+    Q.prog_hi.clear();
+    Q.vals.clear();
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<GroupOut> &gs = outs[pass];
+        if (pass == 1 && gs.empty()) break;
+        std::vector<int32_t> &P = pass ? Q.prog_hi : Q.prog;
+        // constant-weight groups first (fixed-size records), then window groups
+        std::stable_partition(gs.begin(), gs.end(), [](const GroupOut &g) { return g.win.empty(); });
+        int nconst = 0;
+        for (auto &g : gs) nconst += g.win.empty() ? 1 : 0;
+        P.insert(P.end(), {nconst, (int32_t)gs.size() - nconst, 0, 0});
+        for (auto &g : gs) {
+            P.insert(P.end(), {(int32_t)g.F, (int32_t)g.win.size(), 0, 0});
+            put_cd(P, g.c);
+            for (auto &w : g.win) {
+                const int nb = std::min(8, nbits - w.first);
+                P.insert(P.end(), {w.first, nb, (int32_t)Q.vals.size(), 0});
+                Q.vals.insert(Q.vals.end(), w.second.begin(), w.second.end());
+            }
         }
     }
     if (Q.vals.empty()) Q.vals.push_back(cd(0, 0));
@@ -820,15 +830,15 @@ struct QubitPlan {
     DevBuf prog, vals;
 };
 
-// Shift the value offsets of a compiled qubit program by vb.
+// Shift the table offsets of a compiled qubit program by vb.
 static void rebase_qubit_prog(std::vector<int32_t> &p, int32_t vb) {
     if (p.empty()) return;
-    size_t i = 1;
-    for (int g = 0; g < p[0]; ++g) {
+    const int nconst = p[0], nwg = p[1];
+    size_t i = 4 + 8 * (size_t)nconst;
+    for (int g = 0; g < nwg; ++g) {
         const int nwin = p[i + 1];
-        if (nwin == 0) p[i + 2] += vb;
-        i += 3;
-        for (int k = 0; k < nwin; ++k, i += 3) p[i + 2] += vb;
+        i += 8;
+        for (int k = 0; k < nwin; ++k, i += 4) p[i + 2] += vb;
     }
 }
 
@@ -881,7 +891,7 @@ static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, std::shared_p
     std::lock_guard<std::mutex> g(op.mu);
     const int tb = g_qubit_tbits;
     const std::string k = "qb:" + std::to_string(op.version) + (trans ? ":T:" : ":N:") + std::to_string((int)dt) +
-                          ":" + std::to_string(tb);
+                          ":" + std::to_string(tb) + ":" + std::to_string(g_qubit_split);
     auto it = op.qplans.find(k);
     if (it == op.qplans.end()) {
         auto P = std::make_shared<QubitPlan>();
@@ -1723,7 +1733,8 @@ static int *tune_slot(const char *name) {
                {"nt_store", &g_nt_store},         {"copy_variant", &g_copy_variant},
                {"fused_lt", &g_fused_lt},         {"qubit_enable", &g_qubit_enable},
                {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
-               {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap}};
+               {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
+               {"qubit_split", &g_qubit_split}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;

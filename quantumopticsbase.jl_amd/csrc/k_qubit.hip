@@ -18,9 +18,16 @@
 // global memory (coalesced, served by L2 / Infinity Cache).  Work items
 // (state, tile) of all states of a batch share one launch.
 //
-// Program of a state (int32, wave-uniform reads):
-//   [ngroups] then per group [F, nwin, const_off] followed by nwin windows
-//   [shift, nbits, table_off]; nwin == 0 means the weight is vals[const_off].
+// Program of a state (int32, read with wave-uniform scalar loads; records
+// are multiples of 4 ints, so every record starts 16-byte aligned):
+//   header            [nconst, nwgroups, 0, 0]
+//   nconst records    [F, 0, 0, 0, w.re, w.im]   (w as two doubles = 4 ints)
+//                     flip F with a constant, nonzero weight w
+//   nwgroups records  [F, nwin, 0, 0, c.re, c.im] each followed by nwin
+//                     windows [shift, nbits, table_off, 0]; the weight is
+//                     w_F(J) = c + sum_win vals[table_off + ((J >> shift) & (2^nbits - 1))]
+// Constant records have a fixed stride: the record of group g + 1 does not
+// depend on group g and is loaded while group g is applied.
 #include "qob_internal.h"
 
 namespace qob {
@@ -40,17 +47,23 @@ template <typename T> __device__ __forceinline__ T to_t(double2 a) {
     T r; r.x = (typename real_of<T>::type)a.x; r.y = (typename real_of<T>::type)a.y; return r;
 }
 
+// complex weight stored as two doubles in program words r[0..3]
+template <typename T> __device__ __forceinline__ T rec_weight(const int32_t *r) {
+    T w;
+    w.x = (typename real_of<T>::type)__hiloint2double(r[1], r[0]);
+    w.y = (typename real_of<T>::type)__hiloint2double(r[3], r[2]);
+    return w;
+}
+
 // acc += w * x for every element of the lane, w uniform; a real weight
 // needs two FMAs instead of four.  Partner of element k (tile index
 // e = tid + k NTH) under flip F is e ^ F = (tid ^ F) ^ (k NTH): one XOR with a
-// compile-time constant per element.
-template <typename T, int NTH, int EPT, bool CHECK>
-__device__ __forceinline__ void qb_uniform(T wt, const char *basep, uint32_t tpb, int tsize, T (&acc)[EPT]) {
-    const int tid = threadIdx.x;
+// compile-time constant per element, on the byte offset tpb = (tid ^ F) * sizeof(T).
+template <typename T, int NTH, int EPT>
+__device__ __forceinline__ void qb_uniform(T wt, const char *basep, uint32_t tpb, T (&acc)[EPT]) {
     if (wt.y == 0) {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            if (CHECK && tid + k * NTH >= tsize) continue;
             const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
             acc[k].x = fma(wt.x, x.x, acc[k].x);
             acc[k].y = fma(wt.x, x.y, acc[k].y);
@@ -58,71 +71,104 @@ __device__ __forceinline__ void qb_uniform(T wt, const char *basep, uint32_t tpb
     } else {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            if (CHECK && tid + k * NTH >= tsize) continue;
             const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
             acc[k] = cfma(wt, x, acc[k]);
         }
     }
 }
 
-template <typename T, int NTH, int EPT, bool CHECK>
-__device__ __forceinline__ void qb_groups(const int32_t *__restrict__ pg, const T *__restrict__ vals, const T *tile,
-                                          const T *__restrict__ in, uint32_t base, int tb, int tsize,
-                                          T (&acc)[EPT]) {
-    const int tid = threadIdx.x;
-    const int ngroups = pg[0];
-    pg += 1;
-    const uint32_t lowmask = (uint32_t)tsize - 1u;
-    for (int g = 0; g < ngroups; ++g) {
-        const uint32_t F = (uint32_t)pg[0];
-        const int nwin = pg[1];
-        const int coff = pg[2];
-        pg += 3;
-        const bool in_tile = (F >> tb) == 0;
-        // partner addressing: LDS tile, or the partner tile in global memory
-        const uint32_t tpb = ((uint32_t)tid ^ (F & lowmask)) * (uint32_t)sizeof(T);
-        if (nwin == 0) {
-            const T wt = vals[coff];
-            if (wt.x == 0 && wt.y == 0) continue;
-            if (in_tile) {
-                qb_uniform<T, NTH, EPT, CHECK>(wt, (const char *)tile, tpb, tsize, acc);
+// Tile coordinates of the first pass: tile element e is basis state base + e.
+struct LoMap {
+    uint32_t base, lowmask;
+    int tb;
+    __device__ __forceinline__ uint32_t J(uint32_t e) const { return base + e; }
+    __device__ __forceinline__ bool in_tile(uint32_t F) const { return (F >> tb) == 0; }
+    __device__ __forceinline__ uint32_t fq(uint32_t F) const { return F; }
+};
+
+// Tile coordinates of the second pass (strided tile, see k_qubit_hi).
+template <int TBITS> struct HiMap {
+    uint32_t mid, lowmask;  // lowmask = 2^c - 1
+    int c;
+    __device__ __forceinline__ uint32_t J(uint32_t q) const {
+        return ((q >> c) << TBITS) | mid | (q & lowmask);
+    }
+    __device__ __forceinline__ bool in_tile(uint32_t F) const {
+        return ((F & ((1u << TBITS) - 1u)) >> c) == 0;
+    }
+    __device__ __forceinline__ uint32_t fq(uint32_t F) const {
+        return ((F >> TBITS) << c) | (F & ((1u << TBITS) - 1u));
+    }
+};
+
+// acc[k] += w_F(J) * in[J ^ F] over the groups of one program, for the
+// EPT elements e = tid + k NTH of the lane's tile.  Elements past a partial
+// tile compute on unused LDS words and are never stored; partners of a
+// partial tile never leave it (the state is a single tile then).
+template <typename T, int NTH, int EPT, bool LO, class Map>
+__device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *__restrict__ vals, const T *tile,
+                                       const T *__restrict__ in, const Map &m, T (&acc)[EPT]) {
+    const uint32_t tid = threadIdx.x;
+    const int nconst = pg[0];
+    const int nwg = pg[1];
+    const int32_t *r = pg + 4;
+    if (nconst > 0) {
+        uint32_t F = (uint32_t)r[0];
+        T wt = rec_weight<T>(r + 4);
+        for (int g = 0; g < nconst; ++g) {
+            const int32_t *rn = r + 8 * (g + 1 < nconst ? g + 1 : g);
+            const uint32_t Fn = (uint32_t)rn[0];
+            const T wn = rec_weight<T>(rn + 4);
+            if (m.in_tile(F)) {
+                qb_uniform<T, NTH, EPT>(wt, (const char *)tile, (tid ^ m.fq(F)) * (uint32_t)sizeof(T), acc);
+            } else if (LO) {
+                // partner tile in global memory: (base + e) ^ F = (base ^ Fhigh) + (e ^ Flow)
+                const char *gp = (const char *)(in + (m.J(0) ^ (F & ~m.lowmask)));
+                qb_uniform<T, NTH, EPT>(wt, gp, (tid ^ (F & m.lowmask)) * (uint32_t)sizeof(T), acc);
             } else {
-                const char *gp = (const char *)(in + (base ^ (F & ~lowmask)));
-                qb_uniform<T, NTH, EPT, CHECK>(wt, gp, tpb, tsize, acc);
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) acc[k] = cfma(wt, in[m.J(tid + k * NTH) ^ F], acc[k]);
+            }
+            F = Fn;
+            wt = wn;
+        }
+    }
+    r += 8 * nconst;
+    for (int g = 0; g < nwg; ++g) {
+        const uint32_t F = (uint32_t)r[0];
+        const int nwin = r[1];
+        const T c0 = rec_weight<T>(r + 4);
+        r += 8;
+        T wt[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) wt[k] = c0;
+        for (int q = 0; q < nwin; ++q, r += 4) {
+            const int sh = r[0], toff = r[2];
+            const uint32_t mask = (1u << r[1]) - 1u;
+#pragma unroll
+            for (int k = 0; k < EPT; ++k)
+                wt[k] = cadd(wt[k], vals[toff + (int)((m.J(tid + k * NTH) >> sh) & mask)]);
+        }
+        // a zero weight must not propagate a NaN / Inf partner (rows absent
+        // from the factors): select instead of multiply
+        if (m.in_tile(F)) {
+            const uint32_t tpb = (tid ^ m.fq(F)) * (uint32_t)sizeof(T);
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                const T x = *(const T *)((const char *)tile + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+                const T v = cfma(wt[k], x, acc[k]);
+                const bool nz = wt[k].x != 0 || wt[k].y != 0;
+                acc[k].x = nz ? v.x : acc[k].x;
+                acc[k].y = nz ? v.y : acc[k].y;
             }
         } else {
-            const char *basep = (const char *)(in + (base ^ (F & ~lowmask)));
-            T wt[EPT];
 #pragma unroll
-            for (int k = 0; k < EPT; ++k) wt[k] = czero<T>();
-            for (int q = 0; q < nwin; ++q) {
-                const int sh = pg[0], nb = pg[1], toff = pg[2];
-                pg += 3;
-                const uint32_t mask = (1u << nb) - 1u;
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) {
-                    const uint32_t J = base + (uint32_t)(tid + k * NTH);
-                    wt[k] = cadd(wt[k], vals[toff + (int)((J >> sh) & mask)]);
-                }
-            }
-            if (in_tile) {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) {
-                    if (CHECK && tid + k * NTH >= tsize) continue;
-                    if (wt[k].x != 0 || wt[k].y != 0) {
-                        const T x = *(const T *)((const char *)tile + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
-                        acc[k] = cfma(wt[k], x, acc[k]);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) {
-                    if (CHECK && tid + k * NTH >= tsize) continue;
-                    if (wt[k].x != 0 || wt[k].y != 0) {
-                        const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
-                        acc[k] = cfma(wt[k], x, acc[k]);
-                    }
-                }
+            for (int k = 0; k < EPT; ++k) {
+                const T x = in[m.J(tid + k * NTH) ^ F];
+                const T v = cfma(wt[k], x, acc[k]);
+                const bool nz = wt[k].x != 0 || wt[k].y != 0;
+                acc[k].x = nz ? v.x : acc[k].x;
+                acc[k].y = nz ? v.y : acc[k].y;
             }
         }
     }
@@ -182,10 +228,8 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
         T acc[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
-        if (tsize == (1 << TBITS))
-            qb_groups<T, NTH, EPT, false>(prog + S.prog_off, vals, tile, in, base, tb, tsize, acc);
-        else
-            qb_groups<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, in, base, tb, tsize, acc);
+        const LoMap m{base, (uint32_t)tsize - 1u, tb};
+        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, in, m, acc);
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
@@ -236,70 +280,23 @@ __global__ void __launch_bounds__(NTH) k_qubit_hi(const QubitStateDev *__restric
         }
         const int h = S.nbits - TBITS;
         const int c = TBITS - h;
-        const uint32_t lowmask = (1u << c) - 1u;
-        const uint32_t mid = (uint32_t)w.tile << c;
+        const HiMap<TBITS> m{(uint32_t)w.tile << c, (1u << c) - 1u, c};
         const T *__restrict__ in = (const T *)S.in;
         T *__restrict__ out = (T *)S.out;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const uint32_t q = (uint32_t)(tid + k * NTH);
-            tile[q] = in[((q >> c) << TBITS) | mid | (q & lowmask)];
+            tile[q] = in[m.J(q)];
         }
         __syncthreads();
         T acc[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
-        const int32_t *pg = prog + S.prog_hi_off;
-        const int ngroups = pg[0];
-        pg += 1;
-        for (int g = 0; g < ngroups; ++g) {
-            const uint32_t F = (uint32_t)pg[0];
-            const int nwin = pg[1];
-            const int coff = pg[2];
-            pg += 3;
-            const uint32_t Flo = F & ((1u << TBITS) - 1u);
-            const bool in_tile = (Flo >> c) == 0;
-            const uint32_t Fq = ((F >> TBITS) << c) | Flo;  // the flip in tile coordinates
-            T wt[EPT];
-            if (nwin == 0) {
-                const T u = vals[coff];
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) wt[k] = u;
-            } else {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) wt[k] = czero<T>();
-                for (int qq = 0; qq < nwin; ++qq) {
-                    const int sh = pg[0], nb = pg[1], toff = pg[2];
-                    pg += 3;
-                    const uint32_t mask = (1u << nb) - 1u;
-#pragma unroll
-                    for (int k = 0; k < EPT; ++k) {
-                        const uint32_t q = (uint32_t)(tid + k * NTH);
-                        const uint32_t J = ((q >> c) << TBITS) | mid | (q & lowmask);
-                        wt[k] = cadd(wt[k], vals[toff + (int)((J >> sh) & mask)]);
-                    }
-                }
-            }
-            if (in_tile) {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) {
-                    const uint32_t q = (uint32_t)(tid + k * NTH);
-                    if (wt[k].x != 0 || wt[k].y != 0) acc[k] = cfma(wt[k], tile[q ^ Fq], acc[k]);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) {
-                    const uint32_t q = (uint32_t)(tid + k * NTH);
-                    const uint32_t J = ((q >> c) << TBITS) | mid | (q & lowmask);
-                    if (wt[k].x != 0 || wt[k].y != 0) acc[k] = cfma(wt[k], in[J ^ F], acc[k]);
-                }
-            }
-        }
+        qb_run<T, NTH, EPT, false>(prog + S.prog_hi_off, vals, tile, in, m, acc);
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const uint32_t q = (uint32_t)(tid + k * NTH);
-            const uint32_t J = ((q >> c) << TBITS) | mid | (q & lowmask);
+            const uint32_t J = m.J((uint32_t)(tid + k * NTH));
             T r = cmul(alpha, acc[k]);
             if (!beta_zero) r = cfma(beta, out[J], r);
             out[J] = r;
@@ -327,9 +324,9 @@ __global__ void __launch_bounds__(NTH) k_qubit_axpy(const QubitStateDev *__restr
 
 }  // namespace
 
-int g_qubit_nth = 256;
+int g_qubit_nth = 512;
 int g_qubit_remap = 1;
-int g_qubit_grid = 0;  // 0: 4 workgroups per CU
+int g_qubit_grid = 0;  // 0: 5 workgroups per CU
 
 static unsigned qubit_grid(int nitems) {
     int dev = 0, ncu = 256;
@@ -337,11 +334,11 @@ static unsigned qubit_grid(int nitems) {
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
     }
-    const int64_t g = g_qubit_grid > 0 ? g_qubit_grid : (int64_t)ncu * 4;
+    const int64_t g = g_qubit_grid > 0 ? g_qubit_grid : (int64_t)ncu * 5;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, nitems));
 }
 
-int g_qubit_tbits = 12;
+int g_qubit_tbits = 11;
 
 static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
                                    int tbits, const int32_t *prog, const void *vals, double2 alpha, double2 beta,

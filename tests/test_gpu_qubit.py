@@ -218,3 +218,48 @@ def test_batch_graph_replays_match_eager():
             O.mul_ket(k, H, x, 1, 0)
             x = x + c * k
         assert O.rel_frobenius(a.to_numpy(), x) < 1e-12
+
+
+def _tune_get(name):
+    import ctypes as C
+    v = C.c_int64()
+    _lib.call("qob_tune_get", name.encode(), C.byref(v))
+    return v.value
+
+
+@pytest.mark.parametrize("split,tb,nth", [(1, 10, 256), (1, 12, 512), (0, 10, 1024), (0, 13, 512)])
+def test_qubit_tile_configs_vs_oracle(split, tb, nth):
+    """Every tile size / thread count, with the high-bit flips either in a
+    second strided pass (split=1) or read from partner tiles (split=0)."""
+    saved = {k: _tune_get(k) for k in ("qubit_split", "qubit_tbits", "qubit_nth")}
+    try:
+        _lib.call("qob_tune_set", b"qubit_split", split)
+        _lib.call("qob_tune_set", b"qubit_tbits", tb)
+        _lib.call("qob_tune_set", b"qubit_nth", nth)
+        g = np.random.default_rng(300 + tb)
+        L = 15
+        Hq, Ho, B = build_pair(L, random_pauli_sum(g, L, 24, maxspan=4))
+        D = 2 ** L
+        al, be = 0.4 + 0.3j, -0.2j
+        psi, r0 = cg(g, D), cg(g, D)
+        want = r0.copy()
+        O.mul_ket(want, Ho, psi, al, be)
+        r = Q.Ket(B, r0)
+        Q.mul_(r, Hq, Q.Ket(B, psi), al, be)
+        assert O.rel_frobenius(r.to_numpy(), want) < 1e-12
+        # batch of chains straddling the tile size
+        Ls = [tb - 2, tb + 1, 2 * tb - 1 if 2 * tb - 1 <= 16 else 16]
+        specs, ops, psis = _tfim_states(Ls, 40 + tb)
+        bt = Q.Batch(ops)
+        ins = [Q.DeviceArray.from_numpy(p) for p in psis]
+        outs = [Q.DeviceArray.from_numpy(np.full(2 ** L_, 0.5 - 0.5j)) for L_ in Ls]
+        bt.bind(0, ins, outs)
+        bt.apply(0, al, be, 0.1)
+        for spec, p, o, L_ in zip(specs, psis, outs, Ls):
+            want = np.full(2 ** L_, 0.5 - 0.5j)
+            O.mul_ket(want, _oracle_of(spec), p, al, be)
+            want = want + 0.1 * p
+            assert O.rel_frobenius(o.to_numpy(), want) < 1e-12
+    finally:
+        for k, v in saved.items():
+            _lib.call("qob_tune_set", k.encode(), v)
