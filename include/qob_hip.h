@@ -91,6 +91,19 @@ qob_status qob_stream_create(qob_ctx ctx, qob_stream *out);
 qob_status qob_stream_destroy(qob_ctx ctx, qob_stream s);
 qob_status qob_stream_synchronize(qob_ctx ctx, qob_stream s);
 
+/* Events on a stream: completion polling and elapsed time, for callers that
+ * overlap several streams (the serving loop).  No reference counterpart:
+ * the reference's mul! is synchronous on the caller's task. */
+typedef void *qob_event;
+qob_status qob_event_create(qob_ctx ctx, qob_event *out);
+qob_status qob_event_record(qob_ctx ctx, qob_event ev, qob_stream s);
+/* *done = 1 when all work recorded before the event has finished, else 0 */
+qob_status qob_event_query(qob_event ev, int *done);
+qob_status qob_event_synchronize(qob_event ev);
+/* milliseconds between two recorded, completed events */
+qob_status qob_event_elapsed(qob_event start, qob_event stop, float *ms);
+qob_status qob_event_destroy(qob_event ev);
+
 /* -------------------------------------------------------------- operators
  * Handles are reference counted: a LazyTensor or LazySum keeps its factors
  * alive; qob_op_release drops the caller's reference. */

@@ -1454,6 +1454,53 @@ qob_status qob_stream_synchronize(qob_ctx ctx, qob_stream s) {
     return QOB_OK;
 }
 
+// ---------------------------------------------------------------- events
+qob_status qob_event_create(qob_ctx ctx, qob_event *out) {
+    if (!ctx || !out) return fail(QOB_EARG, "null argument");
+    QOB_HIP(hipSetDevice(ctx->device));
+    hipEvent_t e = nullptr;
+    QOB_HIP(hipEventCreate(&e));
+    *out = (qob_event)e;
+    return QOB_OK;
+}
+
+qob_status qob_event_record(qob_ctx ctx, qob_event ev, qob_stream s) {
+    if (!ctx || !ev) return fail(QOB_EARG, "null argument");
+    QOB_HIP(hipEventRecord((hipEvent_t)ev, strm(ctx, s)));
+    return QOB_OK;
+}
+
+qob_status qob_event_query(qob_event ev, int *done) {
+    if (!ev || !done) return fail(QOB_EARG, "null argument");
+    const hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipSuccess) {
+        *done = 1;
+        return QOB_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return QOB_OK;
+    }
+    return fail(QOB_EHIP, "hipEventQuery: %s", hipGetErrorString(e));
+}
+
+qob_status qob_event_synchronize(qob_event ev) {
+    if (!ev) return fail(QOB_EARG, "null event");
+    QOB_HIP(hipEventSynchronize((hipEvent_t)ev));
+    return QOB_OK;
+}
+
+qob_status qob_event_elapsed(qob_event start, qob_event stop, float *ms) {
+    if (!start || !stop || !ms) return fail(QOB_EARG, "null argument");
+    QOB_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return QOB_OK;
+}
+
+qob_status qob_event_destroy(qob_event ev) {
+    if (ev) (void)hipEventDestroy((hipEvent_t)ev);
+    return QOB_OK;
+}
+
 // ------------------------------------------------------------- operators
 static qob_op wrap(std::shared_ptr<OpNode> n) {
     auto *h = new qob_op_s();

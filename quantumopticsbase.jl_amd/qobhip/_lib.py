@@ -81,6 +81,12 @@ _PROTOS = {
     "qob_stream_create": (_i32, [_vp, _P(_vp)]),
     "qob_stream_destroy": (_i32, [_vp, _vp]),
     "qob_stream_synchronize": (_i32, [_vp, _vp]),
+    "qob_event_create": (_i32, [_vp, _P(_vp)]),
+    "qob_event_record": (_i32, [_vp, _vp, _vp]),
+    "qob_event_query": (_i32, [_vp, _P(_i32)]),
+    "qob_event_synchronize": (_i32, [_vp]),
+    "qob_event_elapsed": (_i32, [_vp, _vp, _P(C.c_float)]),
+    "qob_event_destroy": (_i32, [_vp]),
     "qob_sparse_create": (_i32, [_vp, _i64, _i64, _P(_i64), _P(_i64), _P(qob_c64), _P(_vp)]),
     "qob_dense_create": (_i32, [_vp, _i64, _i64, _P(qob_c64), _P(_vp)]),
     "qob_eye_create": (_i32, [_vp, _i64, _i64, _P(_vp)]),

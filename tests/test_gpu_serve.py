@@ -1,0 +1,88 @@
+"""Serving loop on the GPU: every job's result is the same alone and in the
+mix, equals the oracle's step sequence, and admission respects the HBM budget."""
+import copy
+
+import numpy as np
+import pytest
+
+import qob_oracle as O
+from qobhip import serve as S
+from qobhip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+STEPS, CHUNK, C_STEP = 8, 4, -0.01j
+
+
+def _oracle_ops(classes):
+    out = {}
+    for cl in classes:
+        if cl.kind == "ket":
+            out[cl.name] = {}
+            for L, op in cl.ops.items():
+                sh = (2,) * L
+                spec = W.tfim_chain(L, 100 + L) if cl.name == "tfim" else W.spin_chain(L, 1)
+                out[cl.name][L] = O.OLazySum(2 ** L, 2 ** L, [c for c, _, _ in spec.terms],
+                                             [O.OLazyTensor(sh, sh, tuple(i), tuple(O.OSparse.from_scipy(f) for f in fs),
+                                                            1.0) for _, i, fs in spec.terms])
+        elif cl.name == "fock_rho":
+            out[cl.name] = O.OSparse.from_scipy(W.fock_fock_spin(5, 1.5, 2).H)
+        else:
+            facs = W.dense_lazytensor_factors((16, 16), 3)
+            out[cl.name] = O.OLazyTensor((16, 16), (16, 16), (1, 2), tuple(O.ODense(f) for f in facs), 1.0)
+    return out
+
+
+def _oracle_result(cl, oop, req):
+    if cl.kind == "ket":
+        x = req.payload.astype(np.complex128)
+        for _ in range(STEPS):
+            k = np.zeros_like(x)
+            O.mul_ket(k, oop[req.L], x, 1, 0)
+            x = x + C_STEP * k
+        return x / np.linalg.norm(x)
+    x = cl.template.to_numpy().astype(np.complex128).reshape(cl.n, cl.n, order="F")
+    x = x * (1.0 + 0.5 * np.random.Generator(np.random.PCG64(req.seed)).uniform())
+    for _ in range(STEPS):
+        k = np.zeros_like(x)
+        O.mul_op_left(k, oop, x, 1, 0)
+        x = x + C_STEP * k
+    return x / np.trace(x)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 1e-5)])
+def test_alone_equals_in_mix_and_oracle(dtype, tol):
+    classes = S.make_classes(dtype, small=True)
+    reqs = S.make_requests(classes, 16, rate=0, seed=3, dtype=dtype)
+    srv = S.Server(classes, dtype, steps=STEPS, chunk=CHUNK, c=C_STEP, keep_states=True, rho_slots=3)
+    mix = srv.run(copy.deepcopy(reqs))
+    assert all(r.status == "done" for r in mix)
+    oops = _oracle_ops(classes)
+    byname = {c.name: c for c in classes}
+    for r, m in zip(reqs, mix):
+        alone = srv.run([copy.deepcopy(r)])[0]
+        assert alone.status == "done"
+        assert O.rel_frobenius(m.state, alone.state) <= tol, (r.rid, r.cls)
+        want = _oracle_result(byname[r.cls], oops[r.cls], r)
+        assert O.rel_frobenius(m.state.astype(np.complex128).reshape(want.shape), want) < 10 * tol, (r.rid, r.cls)
+
+
+def test_budget_admission_and_rejection():
+    dtype = np.complex128
+    classes = S.make_classes(dtype, small=True)
+    reqs = S.make_requests(classes, 24, rate=2000, seed=9, dtype=dtype)
+    item = 16
+    big = max(2 * (2 ** r.L if r.L else {c.name: c for c in classes}[r.cls].n ** 2) * item for r in reqs)
+    budget = 2 * big
+    srv = S.Server(classes, dtype, budget_bytes=budget, steps=STEPS, chunk=CHUNK, rho_slots=4)
+    res = srv.run(reqs)
+    assert all(r.status == "done" for r in res)
+    assert 0 < srv.peak <= budget
+    # a job larger than the budget is rejected, the others still complete
+    srv2 = S.Server(classes, dtype, budget_bytes=big - 1, steps=STEPS, chunk=CHUNK)
+    res2 = srv2.run(S.make_requests(classes, 12, rate=0, seed=9, dtype=dtype))
+    sizes = [srv2._footprint(r) for r in S.make_requests(classes, 12, rate=0, seed=9, dtype=dtype)]
+    for r, sz in zip(res2, sizes):
+        assert r.status == ("rejected" if sz > big - 1 else "done")
+    s = S.summarize(res)
+    assert s["done"] == 24 and s["p99_ms"] >= s["p50_ms"] > 0
