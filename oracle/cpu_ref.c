@@ -9,6 +9,12 @@
  *                         gemm_sp_dense_big :25-47 (j outer, CSC inner)
  *   qobref_gemm_dense_sp  gemm!(alpha, B, M::SparseMatrixCSC, beta, result)
  *                         src/sparsematrix.jl:115-146 (column axpys)
+ *   qobref_lazysum_ket    mul!(result::Ket, H::LazySum, psi::Ket, alpha, beta)
+ *                         for LazySums of all-sparse square LazyTensors:
+ *                         src/operators_lazysum.jl:189-200 (term 1 takes beta,
+ *                         the others beta = 1), each term through
+ *                         _gemm_puresparse (src/operators_lazytensor.jl:729-739)
+ *                         and the recursion _gemm_recursive_lazy_dense (:652-685)
  * Complex numbers are interleaved doubles, matrices column-major, CSC index
  * arrays 1-based int64 exactly as Julia holds them.  Single-threaded, like the
  * reference's hand-written loops.
@@ -57,5 +63,74 @@ void qobref_gemm_dense_sp(double alpha_re, double alpha_im, int64_t rows, int64_
             const cplx *bcol = B + (rowval[i] - 1) * rows;
             for (int64_t j = 0; j < rows; ++j) rc[j] += mi * bcol[j];
         }
+    }
+}
+
+/* One term of a LazySum: factor f of the term sits at site fsite[f] (1-based)
+ * with CSC arrays at the given offsets of the pooled arrays. */
+typedef struct {
+    int nsites;
+    const int64_t *shape;    /* per-site dimension */
+    const int64_t *strides;  /* column-major strides, site 1 fastest */
+    const int *fac_of_site;  /* site (0-based) -> factor index in the pools, or -1 */
+    const int64_t *fdim, *fcp_off, *fnz_off;
+    const int64_t *colptr, *rowval;
+    const cplx *nzval;
+    const cplx *psi;
+    cplx *result;
+} term_ctx;
+
+/* src/operators_lazytensor.jl:652-685 with 1-based K, J */
+static void rec_lazy_dense(const term_ctx *t, int i_k, int64_t K, int64_t J, cplx val) {
+    if (i_k > t->nsites) {
+        t->result[J - 1] += val * t->psi[K - 1];
+        return;
+    }
+    const int64_t st = t->strides[i_k - 1];
+    const int f = t->fac_of_site[i_k - 1];
+    if (f >= 0) {
+        const int64_t *cp = t->colptr + t->fcp_off[f];
+        const int64_t *rv = t->rowval + t->fnz_off[f];
+        const cplx *nz = t->nzval + t->fnz_off[f];
+        for (int64_t k = 1; k <= t->fdim[f]; ++k) {
+            const int64_t K_ = K + st * (k - 1);
+            for (int64_t jp = cp[k - 1]; jp < cp[k]; ++jp) {
+                const int64_t j = rv[jp - 1];
+                rec_lazy_dense(t, i_k + 1, K_, J + st * (j - 1), val * nz[jp - 1]);
+            }
+        }
+        return;
+    }
+    for (int64_t k = 1; k <= t->shape[i_k - 1]; ++k)
+        rec_lazy_dense(t, i_k + 1, K + st * (k - 1), J + st * (k - 1), val);
+}
+
+/* coef[t] = LazySum coefficient times the LazyTensor factor of term t;
+ * term t owns factors term_off[t] .. term_off[t+1]-1 of the pools. */
+void qobref_lazysum_ket(double alpha_re, double alpha_im, int nsites, const int64_t *shape, int nterms,
+                        const double *coef, const int64_t *term_off, const int64_t *fsite, const int64_t *fdim,
+                        const int64_t *fcp_off, const int64_t *fnz_off, const int64_t *colptr,
+                        const int64_t *rowval, const cplx *nzval, const cplx *psi, double beta_re, double beta_im,
+                        cplx *result) {
+    const cplx alpha = alpha_re + alpha_im * I, beta = beta_re + beta_im * I;
+    int64_t strides[64];
+    int fac_of_site[64];
+    int64_t D = 1;
+    for (int i = 0; i < nsites && i < 64; ++i) {
+        strides[i] = D;
+        D *= shape[i];
+    }
+    if (nterms == 0) {
+        beta_prep(beta, result, D);  /* empty LazySum: zero operator */
+        return;
+    }
+    for (int t = 0; t < nterms; ++t) {
+        beta_prep(t == 0 ? beta : 1, result, D);
+        for (int i = 0; i < nsites; ++i) fac_of_site[i] = -1;
+        for (int64_t f = term_off[t]; f < term_off[t + 1]; ++f) fac_of_site[fsite[f] - 1] = (int)f;
+        term_ctx c = {nsites, shape, strides, fac_of_site, fdim, fcp_off, fnz_off, colptr, rowval, nzval, psi,
+                      result};
+        const cplx ct = coef[2 * t] + coef[2 * t + 1] * I;
+        rec_lazy_dense(&c, 1, 1, 1, alpha * ct);
     }
 }

@@ -10,6 +10,8 @@
 // does not depend on scheduling.
 #include "qob_internal.h"
 
+#include <atomic>
+
 namespace qob {
 
 namespace {
@@ -210,6 +212,17 @@ int reduce_blocks(int64_t n) {
     if (g > 1024) g = 1024;
     if (g < 1) g = 1;
     return (int)g;
+}
+
+int device_cu_count() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n > 0) return n;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cache[dev].store(n, std::memory_order_relaxed);
+    return n;
 }
 
 hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t s) {

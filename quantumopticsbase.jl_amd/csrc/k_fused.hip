@@ -25,6 +25,8 @@
 //   f64 D   D[(lane/16) + 4 r][lane % 16]   register r
 #include "qob_internal.h"
 
+#include <atomic>
+
 namespace qob {
 
 namespace {
@@ -52,13 +54,13 @@ __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 
 constexpr int NTH = 512;   // 8 waves
 constexpr int NW = NTH / 64;
-constexpr int GPW = 2;     // fibre groups in flight per wave
 
 // Each wave owns groups of 16 fibres; for a group it computes every 16-row
 // output tile (ntm = d/16 of them) before writing the group back, so groups
 // never overlap and no barrier is needed between compute and write-back.
-// MAXNTM: d/16 of the widest factor; EPT: column elements per lane.
-template <typename R, int MAXNTM, int EPT>
+// MAXNTM: d/16 of the widest factor; EPT: column elements per lane; GPW:
+// fibre groups in flight per wave (more MFMA chains in flight, more registers).
+template <typename R, int MAXNTM, int EPT, int GPW>
 __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *__restrict__ in,
                                                    cplx<R> *__restrict__ out, int64_t ncols, double2 alpha_,
                                                    double2 beta_, int beta_zero) {
@@ -196,6 +198,8 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 
 }  // namespace
 
+int g_fused_gpw = 2;  // fibre groups in flight per wave: 1, 2 or 4
+
 // LDS bytes for a column of D elements (padded planes)
 static size_t fused_lds(qob_dtype dt, int D, int dmax) {
     const size_t r = (dt == QOB_C64 ? 8 : 4);
@@ -233,22 +237,28 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
     const int maxd = fused_dmax(M);
     const size_t lds = fused_lds(dt, M.D, maxd);
     const int ept = (M.D + NTH - 1) / NTH;
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
-    }
+    const int ncu = device_cu_count();
     const int64_t per_cu = std::max<int64_t>(1, (160 * 1024) / (int64_t)lds);
     const unsigned g = (unsigned)std::min<int64_t>(ncols, (int64_t)ncu * per_cu);
-#define QOB_FU(RR, NTM, EP)                                                                                  \
+#define QOB_FU3(RR, NTM, EP, GP)                                                                             \
     do {                                                                                                     \
-        auto kfn = k_lt_fused<RR, NTM, EP>;                                                                  \
-        hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                            (int)lds);                                                       \
-        if (e_ != hipSuccess) return e_;                                                                     \
+        auto kfn = k_lt_fused<RR, NTM, EP, GP>;                                                              \
+        static std::atomic<int> lds_set{0}; /* raise the limit once per kernel */                            \
+        if (!lds_set.load()) {                                                                               \
+            hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                160 * 1024);                                                 \
+            if (e_ != hipSuccess) return e_;                                                                 \
+            lds_set.store(1);                                                                                \
+        }                                                                                                    \
         hipLaunchKernelGGL(kfn, dim3(g), dim3(NTH), lds, s, M, (const cplx<RR> *)in, (cplx<RR> *)out, ncols, \
                            alpha, beta, bz);                                                                 \
         return hipGetLastError();                                                                            \
+    } while (0)
+#define QOB_FU(RR, NTM, EP)                                   \
+    do {                                                      \
+        if (g_fused_gpw == 1) QOB_FU3(RR, NTM, EP, 1);       \
+        else if (g_fused_gpw == 4) QOB_FU3(RR, NTM, EP, 4);  \
+        else QOB_FU3(RR, NTM, EP, 2);                        \
     } while (0)
     const int ntm = maxd / 16;
     if (dt == QOB_C64) {
@@ -262,6 +272,7 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
     }
     return hipErrorInvalidValue;
 #undef QOB_FU
+#undef QOB_FU3
     return hipGetLastError();
 }
 

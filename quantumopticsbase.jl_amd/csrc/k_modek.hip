@@ -25,10 +25,10 @@ template <typename T> __device__ __forceinline__ T to_t(double2 a) {
 }
 
 template <typename T, int MAXD>
-__global__ void __launch_bounds__(256) k_modek(const T *__restrict__ A, int d_out, int d_in,
+__global__ void __launch_bounds__(512) k_modek(const T *__restrict__ A, int d_out, int d_in,
                                                const T *__restrict__ in, T *__restrict__ out, int64_t n1,
                                                int64_t n3, double2 alpha_, double2 beta_, int beta_zero) {
-    const int64_t f = blockIdx.x * (int64_t)256 + threadIdx.x;
+    const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (f >= n1 * n3) return;
     const int64_t i3 = f / n1, i1 = f - i3 * n1;
     const T *ib = in + i1 + i3 * n1 * d_in;
@@ -50,6 +50,8 @@ __global__ void __launch_bounds__(256) k_modek(const T *__restrict__ A, int d_ou
 
 }  // namespace
 
+int g_modek_nth = 256;  // fibres (lanes) per workgroup: 64, 128, 256 or 512
+
 bool modek_supported(int64_t d_out, int64_t d_in) { return d_in <= 64 && d_out >= 1 && d_in >= 1; }
 
 hipError_t launch_modek(qob_dtype dt, const void *A, int64_t d_out, int64_t d_in, const void *in, void *out,
@@ -58,9 +60,10 @@ hipError_t launch_modek(qob_dtype dt, const void *A, int64_t d_out, int64_t d_in
     if (nf == 0 || d_out == 0) return hipSuccess;
     if (!modek_supported(d_out, d_in)) return hipErrorInvalidValue;
     const int bz = (beta.x == 0.0 && beta.y == 0.0);
-    const uint32_t g = (uint32_t)((nf + 255) / 256);
+    const int nth = g_modek_nth;
+    const uint32_t g = (uint32_t)((nf + nth - 1) / nth);
 #define QOB_MK(TT, MD)                                                                                     \
-    hipLaunchKernelGGL((k_modek<TT, MD>), dim3(g), dim3(256), 0, s, (const TT *)A, (int)d_out, (int)d_in, \
+    hipLaunchKernelGGL((k_modek<TT, MD>), dim3(g), dim3(nth), 0, s, (const TT *)A, (int)d_out, (int)d_in, \
                        (const TT *)in, (TT *)out, n1, n3, alpha, beta, bz)
     if (dt == QOB_C64) {
         if (d_in <= 4) QOB_MK(double2, 4);

@@ -329,11 +329,7 @@ int g_qubit_remap = 1;
 int g_qubit_grid = 0;  // 0: 5 workgroups per CU
 
 static unsigned qubit_grid(int nitems) {
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
-    }
+    const int ncu = device_cu_count();
     const int64_t g = g_qubit_grid > 0 ? g_qubit_grid : (int64_t)ncu * 5;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, nitems));
 }

@@ -130,6 +130,8 @@ struct GatherPlanDev {
 // All launchers return hipSuccess or the launch error; they never allocate.
 
 // elementwise (k_blas1.hip)
+// Compute units of the current device, queried once per device.
+int device_cu_count();
 hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t s);
 hipError_t launch_copy(qob_dtype dt, int64_t n, const void *x, void *y, hipStream_t s);
 hipError_t launch_axpy(qob_dtype dt, int64_t n, double2 a, const void *x, void *y, hipStream_t s);

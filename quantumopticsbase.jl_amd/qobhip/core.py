@@ -18,6 +18,7 @@ every ``mul_`` is one call into libqob_hip.so.  Python's ``!``-less naming:
 from __future__ import annotations
 
 import ctypes as C
+import os
 from fractions import Fraction
 from typing import Sequence
 
@@ -58,6 +59,9 @@ def context() -> Context:
     global _CTX
     if _CTX is None:
         _CTX = Context(0)
+        if os.environ.get("QOB_TUNE_FILE", "1") != "":
+            from . import tune
+            tune.load()
     return _CTX
 
 

@@ -14,9 +14,13 @@ Scheduling (one process, one GPU):
     step is fused as x' = x + c H x).  The batch is re-formed at every
     cycle boundary: finished jobs leave, waiting jobs join (continuous
     batching), a cycle being ``chunk`` steps on the Ket stream.
-  * Density-matrix jobs each take one of ``rho_slots`` streams; their whole
-    step sequence is enqueued at admission, so several run concurrently on
-    the device next to the Ket batch.
+  * Density-matrix jobs each take one of ``rho_slots`` streams and run
+    there ``rho_chunk`` steps at a time, concurrently with the Ket batch.  With the default of 2 slots the Ket
+    stream, the auxiliary stream (uploads, retirements) and the slots get
+    hardware queues of their own (4 per process by default): a Ket step
+    never waits behind a queued density-matrix sequence.
+  * Up to ``depth`` Ket cycles are in flight, so the Ket stream does not
+    idle while the host retires and admits jobs.
   * Admission is FIFO per engine and requires the job's device footprint
     (state + step buffer) to fit the HBM budget; a job larger than the
     budget is rejected, never queued forever.
@@ -161,14 +165,15 @@ class _Job:
     bytes: int
     bufs: list = field(default_factory=list)
     steps_done: int = 0
+    last_cycle: int = -1
 
 
 class Server:
     """Runs a list of requests in real time; see the module docstring."""
 
-    def __init__(self, classes: Sequence, dtype=np.complex128, budget_bytes: int = 32 << 30, rho_slots: int = 4,
+    def __init__(self, classes: Sequence, dtype=np.complex128, budget_bytes: int = 32 << 30, rho_slots: int = 2,
                  steps: int = 32, chunk: int = 4, c: complex = -0.01j, keep_states: bool = False,
-                 max_batch: int = 1024):
+                 max_batch: int = 1024, depth: int = 2, rho_chunk: int = 8):
         if chunk % 2 or steps % chunk:
             raise ValueError("chunk must be even and divide steps")
         self.classes = {c.name: c for c in classes}
@@ -179,10 +184,12 @@ class Server:
         self.steps, self.chunk, self.c = steps, chunk, c
         self.keep = keep_states
         self.max_batch = max_batch
+        self.depth = max(1, depth)
+        self.rho_chunk = max(1, rho_chunk)
         self.ctx = context()
         self.pool = _Pool()
         self.ket_stream = new_stream()
-        self.ket_ev = Event()
+        self.aux_stream = new_stream()
         self.rho = [{"stream": new_stream(), "ev": Event(), "job": None} for _ in range(rho_slots)]
         self.used = 0
         self.peak = 0
@@ -198,28 +205,31 @@ class Server:
 
     # --------------------------------------------------------- Ket engine
     def _ket_cycle(self) -> bool:
-        if self._ket_busy and not self.ket_ev.done():
-            return False
-        self._ket_busy = False
+        """Keep up to ``depth`` cycles of the Ket batch in flight.  A cycle is
+        ``chunk`` fused steps of every member; membership changes between
+        cycles (finished jobs leave, admitted jobs join).  Uploads and
+        retirements run on the auxiliary stream, so the Ket stream never
+        waits for the host."""
         worked = False
-        # retire finished jobs
-        keep = []
-        for j in self._ket_active:
-            if j.steps_done < self.steps:
-                keep.append(j)
-                continue
-            x = j.bufs[0]
-            nrm = C.c_double()
-            call("qob_norm", self.ctx.h, self.dt, x.n, x.ptr, C.byref(nrm), self.ket_stream)
-            call("qob_normalize_ket", self.ctx.h, self.dt, x.n, x.ptr, self.ket_stream)
-            host = np.empty(x.n, self.dtype)
-            call("qob_memcpy_dtoh", self.ctx.h, host.ctypes.data_as(C.c_void_p), x.ptr, x.nbytes, self.ket_stream)
-            j.res.value = nrm.value
-            j.res.state = host if self.keep else None
-            self._finish(j)
-            worked = True
-        changed = len(keep) != len(self._ket_active)
-        self._ket_active = keep
+        # cycles that finished: retire the jobs whose last cycle it was
+        while self._ket_inflight and self._ket_inflight[0][0].done():
+            ev, cid, _ = self._ket_inflight.popleft()
+            self._ket_events.append(ev)
+            for j in [j for j in self._ket_pending if j.last_cycle == cid]:
+                self._ket_pending.remove(j)
+                x = j.bufs[0]
+                nrm = C.c_double()
+                call("qob_norm", self.ctx.h, self.dt, x.n, x.ptr, C.byref(nrm), self.aux_stream)
+                call("qob_normalize_ket", self.ctx.h, self.dt, x.n, x.ptr, self.aux_stream)
+                host = np.empty(x.n, self.dtype)
+                call("qob_memcpy_dtoh", self.ctx.h, host.ctypes.data_as(C.c_void_p), x.ptr, x.nbytes,
+                     self.aux_stream)
+                j.res.value = nrm.value
+                j.res.state = host if self.keep else None
+                self._finish(j)
+                worked = True
+        if len(self._ket_inflight) >= self.depth:
+            return worked
         # admit waiting jobs, FIFO, while the budget allows
         while self._ket_wait and len(self._ket_active) < self.max_batch:
             j = self._ket_wait[0]
@@ -230,45 +240,75 @@ class Server:
             n = 2 ** j.req.L
             xa, xb = self.pool.get(n, self.dtype), self.pool.get(n, self.dtype)
             p = np.ascontiguousarray(j.req.payload, dtype=self.dtype)
-            call("qob_memcpy_htod", self.ctx.h, xa.ptr, p.ctypes.data_as(C.c_void_p), xa.nbytes, self.ket_stream)
+            call("qob_memcpy_htod", self.ctx.h, xa.ptr, p.ctypes.data_as(C.c_void_p), xa.nbytes, self.aux_stream)
             j.bufs = [xa, xb]
             self._ket_active.append(j)
-            changed = True
+            worked = True
         if not self._ket_active:
-            if self._batch is not None:
-                self._graveyard.append(self._batch)
-            self._batch = None
             return worked
-        if changed or self._batch is None:
+        members = tuple(j.req.rid for j in self._ket_active)
+        if members != self._ket_members:
             ops = [self.classes[j.req.cls].ops[j.req.L] for j in self._ket_active]
             hs = (C.c_void_p * len(ops))(*[op._handle().value for op in ops])
             h = C.c_void_p()
             call("qob_batch_create", self.ctx.h, len(ops), hs, self.dt, C.byref(h))
-            # a replaced batch is freed after the run: freeing device memory
-            # waits for the whole device, which would stall the other streams
+            # a replaced batch may still be read by a cycle in flight, and
+            # freeing device memory waits for the whole device: keep it
+            # until the end of the run
             if self._batch is not None:
                 self._graveyard.append(self._batch)
             self._batch = _BatchHandle(h)
+            self._ket_members = members
             xa = (C.c_void_p * len(ops))(*[j.bufs[0].ptr.value for j in self._ket_active])
             xb = (C.c_void_p * len(ops))(*[j.bufs[1].ptr.value for j in self._ket_active])
             call("qob_batch_bind", self.ctx.h, h, 0, xa, xb)
             call("qob_batch_bind", self.ctx.h, h, 1, xb, xa)
-        # chunk steps, x' = x + c H x, alternating the two buffers
+        # chunk steps, x' = x + c H x, alternating the two buffers (chunk is
+        # even: every member's state is back in its first buffer afterwards)
         for s in range(self.chunk):
             call("qob_batch_apply", self.ctx.h, self._batch.h, s % 2, c64(self.c), c64(0.0), c64(1.0),
                  self.ket_stream)
-        self.ket_ev.record(self.ket_stream)
-        self._ket_busy = True
+        ev = self._ket_events.pop() if self._ket_events else Event()
+        ev.record(self.ket_stream)
+        self._cycle_id += 1
+        self._ket_inflight.append((ev, self._cycle_id, members))
+        keep = []
         for j in self._ket_active:
             j.steps_done += self.chunk
+            if j.steps_done >= self.steps:
+                j.last_cycle = self._cycle_id
+                self._ket_pending.append(j)
+            else:
+                keep.append(j)
+        self._ket_active = keep
         return True
 
     # --------------------------------------------------------- rho engine
+    def _rho_steps(self, slot, j) -> None:
+        """Enqueue the job's next ``rho_chunk`` steps on the slot's stream."""
+        cl = self.classes[j.req.cls]
+        x, k = j.bufs
+        s = slot["stream"]
+        h = cl.op._handle()
+        nn = cl.n * cl.n
+        n = min(self.rho_chunk, self.steps - j.steps_done)
+        for _ in range(n):
+            call("qob_mul_op_left", self.ctx.h, self.dt, h, x.ptr, cl.n, cl.n, k.ptr, cl.n, c64(1.0), c64(0.0), s)
+            call("qob_axpy", self.ctx.h, self.dt, nn, c64(self.c), k.ptr, x.ptr, s)
+        j.steps_done += n
+        slot["ev"].record(s)
+
     def _rho_cycle(self) -> bool:
+        """Each slot runs one job, ``rho_chunk`` steps at a time: short
+        queues let the Ket batch's launches interleave with long sequences."""
         worked = False
         for slot in self.rho:
             j = slot["job"]
             if j is not None and slot["ev"].done():
+                if j.steps_done < self.steps:
+                    self._rho_steps(slot, j)
+                    worked = True
+                    continue
                 x = j.bufs[0]
                 cl = self.classes[j.req.cls]
                 tr = _lib.qob_c64()
@@ -299,12 +339,7 @@ class Server:
                 scale = 1.0 + 0.5 * np.random.Generator(np.random.PCG64(j.req.seed)).uniform()
                 call("qob_copy", self.ctx.h, self.dt, nn, cl.template.ptr, x.ptr, s)
                 call("qob_scale", self.ctx.h, self.dt, nn, c64(scale), x.ptr, s)
-                h = cl.op._handle()
-                for _ in range(self.steps):
-                    call("qob_mul_op_left", self.ctx.h, self.dt, h, x.ptr, cl.n, cl.n, k.ptr, cl.n, c64(1.0),
-                         c64(0.0), s)
-                    call("qob_axpy", self.ctx.h, self.dt, nn, c64(self.c), k.ptr, x.ptr, s)
-                slot["ev"].record(s)
+                self._rho_steps(slot, j)
                 slot["job"] = j
                 worked = True
         return worked
@@ -330,7 +365,11 @@ class Server:
         self._ket_wait: deque = deque()
         self._rho_wait: deque = deque()
         self._ket_active: List[_Job] = []
-        self._ket_busy = False
+        self._ket_pending: List[_Job] = []
+        self._ket_inflight: deque = deque()
+        self._ket_events: list = []
+        self._ket_members: tuple = ()
+        self._cycle_id = 0
         self._batch = None
         self._graveyard: list = []
         total = len(reqs)
@@ -350,7 +389,7 @@ class Server:
             worked = self._ket_cycle()
             worked = self._rho_cycle() or worked
             if not worked:
-                if reqs and not (self._ket_active or self._ket_wait or self._rho_wait
+                if reqs and not (self._ket_active or self._ket_wait or self._ket_inflight or self._rho_wait
                                  or any(s["job"] for s in self.rho)):
                     time.sleep(max(0.0, min(1e-3, reqs[0].t_arrival - self._clock())))
                 else:
