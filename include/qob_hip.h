@@ -140,6 +140,17 @@ qob_status qob_lazysum_create(qob_ctx ctx, int64_t m, int64_t n, int nterms,
 /* Replace the coefficient vector of a LazySum in place (what
  * TimeDependentSum's set_time! does, src/time_dependent_operator.jl:162-170). */
 qob_status qob_lazysum_set_factors(qob_ctx ctx, qob_op sum, const qob_c64 *factors);
+/* The same from device memory, stream-ordered and usable under stream
+ * capture: factors[t] = dfactors[t], nterms complex doubles in device
+ * memory (update_static_coefficients!, src/time_dependent_operator.jl:286-297,
+ * with the coefficient evaluation left to the caller, e.g. a row of a table
+ * of coefficients over a time grid).  No plan is rebuilt: the plans' values
+ * are linear in the coefficients and a refresh kernel rewrites them.
+ * Supported when every term is an all-sparse LazyTensor (the fused gather
+ * and qubit paths); the plain mode then refuses to apply the sum, and a
+ * later qob_lazysum_set_factors makes the host coefficients current again.
+ * Plans are built on first use outside capture, as for every captured call. */
+qob_status qob_lazysum_set_factors_d(qob_ctx ctx, qob_op sum, const void *dfactors, qob_stream s);
 qob_status qob_op_shape(qob_op a, int64_t *m, int64_t *n);
 qob_status qob_op_release(qob_op a);
 

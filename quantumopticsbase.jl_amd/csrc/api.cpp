@@ -33,7 +33,7 @@ namespace qob {
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
-extern int g_modek_nth, g_fused_gpw;
+extern int g_modek_nth, g_fused_gpw, g_csc_rows;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -328,9 +328,61 @@ static qob_status dense_dev(DenseData &D, int v, qob_dtype dt, const void *&ptr)
 // ---------------------------------------------------------------- op nodes
 enum Kind { K_SPARSE, K_DENSE, K_EYE, K_LT, K_LS };
 
+// Linear map from a LazySum's coefficients to a plan's values: slot s (a
+// value of the plan's table when s < nvals, else a complex double inline in
+// its program at word offset inline_off[s - nvals]) equals
+// sum_p bval[p] * coef[term[p]] over p in [ptr[s], ptr[s+1]).
+struct RefreshMap {
+    int nslots = 0, nvals = 0;
+    DevBuf ptr, term, bval, dst;
+};
+
+// Build the refresh map from the slot values obtained with each unit
+// coefficient vector: cols[t][s] is slot s with coefficient e_t.
+static qob_status build_refresh(const std::vector<std::vector<cd>> &cols, int nvals,
+                                const std::vector<int32_t> &inline_off, RefreshMap &R) {
+    const size_t nslots = (size_t)nvals + inline_off.size();
+    std::vector<int32_t> ptr(1, 0), term;
+    std::vector<double2> bval;
+    for (size_t sl = 0; sl < nslots; ++sl) {
+        for (size_t t = 0; t < cols.size(); ++t) {
+            const cd b = cols[t][sl];
+            if (b != cd(0, 0)) {
+                term.push_back((int32_t)t);
+                bval.push_back(d2(b));
+            }
+        }
+        ptr.push_back((int32_t)term.size());
+    }
+    if (term.empty()) {
+        term.push_back(0);
+        bval.push_back(make_double2(0, 0));
+    }
+    R.nslots = (int)nslots;
+    R.nvals = nvals;
+    QOB_TRY(R.ptr.upload(ptr.data(), ptr.size() * 4));
+    QOB_TRY(R.term.upload(term.data(), term.size() * 4));
+    QOB_TRY(R.bval.upload(bval.data(), bval.size() * sizeof(double2)));
+    std::vector<int32_t> dst = inline_off;
+    if (dst.empty()) dst.push_back(0);
+    QOB_TRY(R.dst.upload(dst.data(), dst.size() * 4));
+    return QOB_OK;
+}
+
+static qob_status run_refresh(const RefreshMap &R, qob_dtype dt, const void *coef, void *vals, void *prog,
+                              hipStream_t s) {
+    hipError_t e = launch_refresh(dt, R.nslots, R.nvals, (const int32_t *)R.ptr.p, (const int32_t *)R.term.p,
+                                  (const double2 *)R.bval.p, (const int32_t *)R.dst.p, (const double2 *)coef, vals,
+                                  (int32_t *)prog, s);
+    if (e != hipSuccess) return fail(QOB_EHIP, "coefficient refresh: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
 struct GatherPlan {
     DevBuf prog, rowptr, koff, vals, site_info;
     GatherPlanDev dev;
+    qob_dtype dt = QOB_C64;
+    std::shared_ptr<RefreshMap> rmap;
 };
 
 struct OpNode {
@@ -352,7 +404,11 @@ struct OpNode {
     std::map<std::string, std::shared_ptr<GatherPlan>> gplans;
     std::map<std::string, std::shared_ptr<SparseData>> sums;
     std::map<std::string, std::shared_ptr<struct QubitPlan>> qplans;
-    uint64_t version = 0;  // bumped when coefficients change
+    uint64_t version = 0;  // bumped when coefficients change on the host
+    // LazySum coefficients held on the device (complex doubles); when
+    // dcoef_on they, not coef, are current and every plan's values follow them
+    DevBuf dcoef;
+    bool dcoef_on = false;
 };
 
 static bool is_pure_sparse_factor(const OpNode &o) {
@@ -652,7 +708,8 @@ static RowLists factor_rows(const OpNode &f, bool trans) {
 // trans = false: acts on a Ket / rows of rho (out space = basis_l);
 // trans = true : transposed, acts on a Bra / columns of rho (out = basis_r).
 static qob_status lazy_plan(OpNode &owner, const std::string &key, const std::vector<OpNode *> &lts,
-                            const std::vector<cd> &coefs, bool trans, qob_dtype dt, GatherPlanDev &out) {
+                            const std::vector<cd> &coefs, bool trans, qob_dtype dt, GatherPlanDev &out,
+                            bool refreshable = false, hipStream_t s = nullptr) {
     std::lock_guard<std::mutex> g(owner.mu);
     const std::string k = key + (trans ? ":T:" : ":N:") + std::to_string((int)dt);
     auto it = owner.gplans.find(k);
@@ -683,6 +740,24 @@ static qob_status lazy_plan(OpNode &owner, const std::string &key, const std::ve
         QOB_TRY(compile_plan(out_shape, in_shape, specs, H));
         auto P = std::make_shared<GatherPlan>();
         QOB_TRY(upload_plan(H, dt, *P));
+        P->dt = dt;
+        if (refreshable) {
+            // values as linear functions of the coefficients (the table
+            // layout does not depend on them)
+            std::vector<std::vector<cd>> cols;
+            for (size_t t = 0; t < specs.size(); ++t) {
+                std::vector<TermSpec> unit = specs;
+                for (size_t q = 0; q < unit.size(); ++q) unit[q].coef = (q == t) ? lts[q]->factor : cd(0, 0);
+                HostPlan U;
+                QOB_TRY(compile_plan(out_shape, in_shape, unit, U));
+                if (U.vals.size() != H.vals.size() || U.koff != H.koff || U.rowptr != H.rowptr)
+                    return fail(QOB_EUNSUPPORTED, "gather plan layout depends on the coefficients");
+                cols.push_back(std::move(U.vals));
+            }
+            P->rmap = std::make_shared<RefreshMap>();
+            QOB_TRY(build_refresh(cols, (int)H.vals.size(), {}, *P->rmap));
+            if (owner.dcoef_on) QOB_TRY(run_refresh(*P->rmap, dt, owner.dcoef.p, P->vals.p, nullptr, s));
+        }
         it = owner.gplans.emplace(k, P).first;
     }
     out = it->second->dev;
@@ -757,10 +832,18 @@ static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitPr
         GroupOut go;
         go.F = gq.first;
         go.c = cd(0, 0);
-        // windows of width <= 8 starting at multiples of 6 where possible
+        // windows of width <= 8 starting at multiples of 6 where possible.
+        // The layout depends only on the factors' entry patterns, never on
+        // the coefficients, so that coefficient changes only rewrite values
+        // (a term whose weight pattern is constant goes to the constant part;
+        // groups are kept even when their weights cancel).
         for (const QT *q : gq.second) {
-            if (q->sites.empty()) {
-                go.c += q->coef;
+            bool flat = true;
+            for (size_t i = 0; i < q->sites.size(); ++i) flat = flat && (q->v[i][0] == q->v[i][1]);
+            if (q->sites.empty() || flat) {
+                cd w = q->coef;
+                for (size_t i = 0; i < q->sites.size(); ++i) w *= q->v[i][0];
+                go.c += w;
                 continue;
             }
             const int m = *std::min_element(q->sites.begin(), q->sites.end());
@@ -777,18 +860,6 @@ static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitPr
                 tab[idx] += w;
             }
         }
-        // fold windows whose table is constant into the constant part
-        for (auto it = go.win.begin(); it != go.win.end();) {
-            bool c = true;
-            for (auto &z : it->second) c = c && (z == it->second[0]);
-            if (c) {
-                go.c += it->second[0];
-                it = go.win.erase(it);
-            } else {
-                ++it;
-            }
-        }
-        if (go.win.empty() && go.c == cd(0, 0)) continue;  // the group vanishes
         outs[(split && (go.F >> tb) != 0) ? 1 : 0].push_back(std::move(go));
     }
     auto put_cd = [](std::vector<int32_t> &P, cd z) {
@@ -829,7 +900,30 @@ struct QubitPlan {
     int tbits = 12;
     int32_t hi_off = -1;  // offset of the high-bit program in prog, or -1
     DevBuf prog, vals;
+    qob_dtype dt = QOB_C64;
+    std::shared_ptr<RefreshMap> rmap;
 };
+
+// Word offsets of the complex doubles held inline in a qubit program (the
+// constant weights), program starting at word base.
+static void qubit_inline_offsets(const std::vector<int32_t> &p, int32_t base, std::vector<int32_t> &off) {
+    if (p.empty()) return;
+    const int nconst = p[0], nwg = p[1];
+    size_t i = 4;
+    for (int g = 0; g < nconst; ++g, i += 8) off.push_back(base + (int32_t)i + 4);
+    for (int g = 0; g < nwg; ++g) {
+        const int nwin = p[i + 1];
+        off.push_back(base + (int32_t)i + 4);
+        i += 8 + 4 * (size_t)nwin;
+    }
+}
+
+static cd inline_value(const std::vector<int32_t> &p, int32_t off) {
+    double d[2];
+    std::memcpy(d, &p[(size_t)off], sizeof(d));
+    return cd(d[0], d[1]);
+}
+
 
 // Shift the table offsets of a compiled qubit program by vb.
 static void rebase_qubit_prog(std::vector<int32_t> &p, int32_t vb) {
@@ -845,7 +939,7 @@ static void rebase_qubit_prog(std::vector<int32_t> &p, int32_t vb) {
 
 // Terms of a LazySum / LazyTensor as qubit terms (trans: transposed factors).
 static bool qubit_terms(OpNode &op, bool trans, std::vector<TermSpec> &specs, std::vector<std::vector<RowLists>> &store,
-                        int &nbits) {
+                        int &nbits, std::vector<cd> *base = nullptr) {
     std::vector<OpNode *> lts;
     std::vector<cd> coefs;
     if (op.kind == K_LT) {
@@ -872,6 +966,7 @@ static bool qubit_terms(OpNode &op, bool trans, std::vector<TermSpec> &specs, st
         const OpNode &lt = *lts[t];
         TermSpec ts;
         ts.coef = coefs[t] * lt.factor;
+        if (base) base->push_back(lt.factor);
         for (size_t i = 0; i < lt.fop.size(); ++i)
             if (lt.fop[i]->kind != K_EYE) store[t].push_back(factor_rows(*lt.fop[i], trans));
         size_t k2 = 0;
@@ -888,7 +983,7 @@ static bool qubit_terms(OpNode &op, bool trans, std::vector<TermSpec> &specs, st
 int g_qubit_enable = 1;
 extern int g_qubit_tbits;
 
-static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, std::shared_ptr<QubitPlan> &out) {
+static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s, std::shared_ptr<QubitPlan> &out) {
     std::lock_guard<std::mutex> g(op.mu);
     const int tb = g_qubit_tbits;
     const std::string k = "qb:" + std::to_string(op.version) + (trans ? ":T:" : ":N:") + std::to_string((int)dt) +
@@ -898,9 +993,10 @@ static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, std::shared_p
         auto P = std::make_shared<QubitPlan>();
         std::vector<TermSpec> specs;
         std::vector<std::vector<RowLists>> store;
+        std::vector<cd> base;
         int nbits = 0;
         QubitProgHost H;
-        if (qubit_terms(op, trans, specs, store, nbits) && compile_qubit(nbits, specs, H, tb)) {
+        if (qubit_terms(op, trans, specs, store, nbits, &base) && compile_qubit(nbits, specs, H, tb)) {
             std::vector<int32_t> all = H.prog;
             if (!H.prog_hi.empty()) {
                 P->hi_off = (int32_t)all.size();
@@ -910,7 +1006,37 @@ static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, std::shared_p
             QOB_TRY(upload_vals(P->vals, H.vals, dt));
             P->nbits = nbits;
             P->tbits = tb;
+            P->dt = dt;
             P->ok = true;
+            if (op.kind == K_LS) {
+                // values as linear functions of the coefficients: compile
+                // once per unit coefficient vector (same layout every time)
+                std::vector<int32_t> off;
+                qubit_inline_offsets(H.prog, 0, off);
+                if (!H.prog_hi.empty()) qubit_inline_offsets(H.prog_hi, P->hi_off, off);
+                std::vector<std::vector<cd>> cols;
+                bool same = true;
+                for (size_t t = 0; t < specs.size() && same; ++t) {
+                    std::vector<TermSpec> unit = specs;
+                    for (size_t q = 0; q < unit.size(); ++q) unit[q].coef = (q == t) ? base[q] : cd(0, 0);
+                    QubitProgHost U;
+                    same = compile_qubit(nbits, unit, U, tb) && U.prog.size() == H.prog.size() &&
+                           U.prog_hi.size() == H.prog_hi.size() && U.vals.size() == H.vals.size();
+                    if (!same) break;
+                    std::vector<int32_t> ua = U.prog;
+                    ua.insert(ua.end(), U.prog_hi.begin(), U.prog_hi.end());
+                    std::vector<cd> col = U.vals;
+                    for (int32_t o : off) col.push_back(inline_value(ua, o));
+                    cols.push_back(std::move(col));
+                }
+                if (same) {
+                    P->rmap = std::make_shared<RefreshMap>();
+                    QOB_TRY(build_refresh(cols, (int)H.vals.size(), off, *P->rmap));
+                    if (op.dcoef_on) QOB_TRY(run_refresh(*P->rmap, dt, op.dcoef.p, P->vals.p, P->prog.p, s));
+                } else if (op.dcoef_on) {
+                    return fail(QOB_EUNSUPPORTED, "qubit plan values are not linear in the coefficients");
+                }
+            }
         }
         it = op.qplans.emplace(k, P).first;
     }
@@ -1210,7 +1336,7 @@ static qob_status apply_qubit(const Apply &ap, OpNode &op, const void *in, void 
     done = false;
     if (!g_qubit_enable || ap.lead != 1) return QOB_OK;
     std::shared_ptr<QubitPlan> P;
-    QOB_TRY(qubit_plan(op, ap.trans, ap.dt, P));
+    QOB_TRY(qubit_plan(op, ap.trans, ap.dt, ap.s, P));
     if (!P->ok) return QOB_OK;
     QubitStateDev S;
     S.nbits = P->nbits;
@@ -1253,6 +1379,8 @@ static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *ou
     const int64_t dout = op_out_dim(op, ap.trans);
     if (op.terms.empty() || is_zero(alpha)) return apply_scale(ap, dout, out, beta);
     if (ap.ctx->mode == QOB_MODE_PLAIN) {
+        if (op.dcoef_on)
+            return fail(QOB_EUNSUPPORTED, "LazySum with device-held coefficients: use the fused mode");
         for (size_t t = 0; t < op.terms.size(); ++t)
             QOB_TRY(apply_op(ap, *op.terms[t], in, out, alpha * op.coef[t], t == 0 ? beta : cd(1, 0)));
         return QOB_OK;
@@ -1279,7 +1407,8 @@ static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *ou
     cd b = beta;
     if (!lts.empty()) {
         GatherPlanDev P;
-        QOB_TRY(lazy_plan(op, "ls" + std::to_string(op.version), lts, lcoef, ap.trans, ap.dt, P));
+        QOB_TRY(lazy_plan(op, "ls" + std::to_string(op.version), lts, lcoef, ap.trans, ap.dt, P,
+                          lts.size() == op.terms.size(), ap.s));
         QOB_TRY(apply_gather(ap, P, op_in_dim(op, ap.trans), dout, in, out, alpha, b));
         b = cd(1, 0);
     }
@@ -1657,6 +1786,12 @@ qob_status qob_lazysum_create(qob_ctx ctx, int64_t m, int64_t n, int nterms, con
         node->coef.push_back(cdv(factors[t]));
         node->terms.push_back(x);
     }
+    {
+        std::vector<double2> dc(node->coef.size());
+        for (size_t t = 0; t < dc.size(); ++t) dc[t] = d2(node->coef[t]);
+        if (dc.empty()) dc.push_back(make_double2(0, 0));
+        QOB_TRY(node->dcoef.upload(dc.data(), dc.size() * sizeof(double2)));
+    }
     *out = wrap(node);
     return QOB_OK;
 }
@@ -1673,6 +1808,37 @@ qob_status qob_lazysum_set_factors(qob_ctx ctx, qob_op sum, const qob_c64 *facto
     n.gplans.clear();
     n.sums.clear();
     n.qplans.clear();
+    n.dcoef_on = false;
+    std::vector<double2> dc(n.coef.size());
+    for (size_t t = 0; t < dc.size(); ++t) dc[t] = d2(n.coef[t]);
+    if (!dc.empty()) QOB_TRY(n.dcoef.upload(dc.data(), dc.size() * sizeof(double2)));
+    return QOB_OK;
+}
+
+qob_status qob_lazysum_set_factors_d(qob_ctx ctx, qob_op sum, const void *dfactors, qob_stream s) {
+    if (!ctx || !sum || sum->node->kind != K_LS) return fail(QOB_EARG, "not a LazySum");
+    OpNode &n = *sum->node;
+    if (n.terms.empty()) return QOB_OK;
+    if (!dfactors) return fail(QOB_EARG, "null factors");
+    for (auto &t : n.terms)
+        if (!lt_pure_sparse(*t))
+            return fail(QOB_EUNSUPPORTED, "device-held coefficients need every term to be an all-sparse LazyTensor");
+    const hipStream_t st = strm(ctx, s);
+    std::lock_guard<std::mutex> g(n.mu);
+    hipError_t e = launch_copy(QOB_C64, (int64_t)n.terms.size(), dfactors, n.dcoef.p, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "coefficient copy: %s", hipGetErrorString(e));
+    for (auto &kv : n.qplans) {
+        QubitPlan &P = *kv.second;
+        if (!P.ok) continue;
+        if (!P.rmap) return fail(QOB_EUNSUPPORTED, "qubit plan without a coefficient map");
+        QOB_TRY(run_refresh(*P.rmap, P.dt, n.dcoef.p, P.vals.p, P.prog.p, st));
+    }
+    for (auto &kv : n.gplans) {
+        GatherPlan &P = *kv.second;
+        if (!P.rmap) return fail(QOB_EUNSUPPORTED, "gather plan without a coefficient map");
+        QOB_TRY(run_refresh(*P.rmap, P.dt, n.dcoef.p, P.vals.p, nullptr, st));
+    }
+    n.dcoef_on = true;
     return QOB_OK;
 }
 
@@ -1783,7 +1949,7 @@ static int *tune_slot(const char *name) {
                {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
                {"qubit_split", &g_qubit_split},   {"modek_nth", &g_modek_nth},
-               {"fused_gpw", &g_fused_gpw}};
+               {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -1800,6 +1966,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_tbits") && (value < 10 || value > 13)) return fail(QOB_EARG, "qubit_tbits must be in 10..13");
     if (!strcmp(name, "modek_nth") && value != 64 && value != 128 && value != 256 && value != 512)
         return fail(QOB_EARG, "modek_nth must be 64, 128, 256 or 512");
+    if (!strcmp(name, "csc_rows") && value != 16 && value != 32 && value != 64)
+        return fail(QOB_EARG, "csc_rows must be 16, 32 or 64");
     if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)
         return fail(QOB_EARG, "fused_gpw must be 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");

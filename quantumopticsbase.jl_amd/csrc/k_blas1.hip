@@ -5,6 +5,9 @@
 //   dot    : dot(x, y) = sum conj(x) y (src/states.jl:80 with dagger)
 //   nrm2   : norm (src/states.jl:89)
 //   trace  : tr (src/operators_dense.jl:189)
+//   refresh: plan values as linear combinations of device-held LazySum
+//            coefficients (update_static_coefficients!,
+//            src/time_dependent_operator.jl:286-297, done on the device)
 // Reductions accumulate in double for both element types and run in two
 // deterministic stages (per-block partials, then one block), so a result
 // does not depend on scheduling.
@@ -28,6 +31,29 @@ __global__ void __launch_bounds__(BS) k_scale(int64_t n, double2 a_, T *x) {
     const bool zero = (a_.x == 0.0 && a_.y == 0.0);
     for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
         x[i] = zero ? czero<T>() : cmul(a, x[i]);
+    }
+}
+
+// slot s = sum_{p in [ptr[s], ptr[s+1])} bval[p] * coef[term[p]], in FP64;
+// slots below nvals are plan values of element type T, the others complex
+// doubles written into the program at word offsets dst[s - nvals]
+template <typename T>
+__global__ void __launch_bounds__(BS) k_refresh(int nslots, int nvals, const int32_t *__restrict__ ptr,
+                                                const int32_t *__restrict__ term, const double2 *__restrict__ bval,
+                                                const int32_t *__restrict__ dst, const double2 *__restrict__ coef,
+                                                T *__restrict__ vals, int32_t *__restrict__ prog) {
+    const int s = blockIdx.x * BS + threadIdx.x;
+    if (s >= nslots) return;
+    double re = 0, im = 0;
+    for (int p = ptr[s]; p < ptr[s + 1]; ++p) {
+        const double2 b = bval[p], c = coef[term[p]];
+        re += b.x * c.x - b.y * c.y;
+        im += b.x * c.y + b.y * c.x;
+    }
+    if (s < nvals) {
+        vals[s] = from_d2<T>(make_double2(re, im));
+    } else {
+        *(double2 *)(prog + dst[s - nvals]) = make_double2(re, im);
     }
 }
 
@@ -229,6 +255,20 @@ hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t
     if (n <= 0) return hipSuccess;
     if (dt == QOB_C64) hipLaunchKernelGGL(k_scale<double2>, grid_for(n), BS, 0, s, n, a, (double2 *)x);
     else hipLaunchKernelGGL(k_scale<float2>, grid_for(n), BS, 0, s, n, a, (float2 *)x);
+    return hipGetLastError();
+}
+
+hipError_t launch_refresh(qob_dtype dt, int nslots, int nvals, const int32_t *ptr, const int32_t *term,
+                          const double2 *bval, const int32_t *dst, const double2 *coef, void *vals, int32_t *prog,
+                          hipStream_t s) {
+    if (nslots <= 0) return hipSuccess;
+    const unsigned g = (unsigned)((nslots + BS - 1) / BS);
+    if (dt == QOB_C64)
+        hipLaunchKernelGGL(k_refresh<double2>, dim3(g), dim3(BS), 0, s, nslots, nvals, ptr, term, bval, dst, coef,
+                           (double2 *)vals, prog);
+    else
+        hipLaunchKernelGGL(k_refresh<float2>, dim3(g), dim3(BS), 0, s, nslots, nvals, ptr, term, bval, dst, coef,
+                           (float2 *)vals, prog);
     return hipGetLastError();
 }
 

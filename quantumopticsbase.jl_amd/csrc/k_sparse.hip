@@ -164,6 +164,36 @@ __global__ void __launch_bounds__(RB) k_csc_right(int64_t rows, int64_t n, const
     }
 }
 
+// ---- right, short stripes: a wave covers RR rows x (64 / RR) columns per
+// step (lane l: row l % RR, column l / RR).  Workgroups of one XCD sweep a
+// stripe of RR rows across all columns, so a column of B re-read by the
+// column delta positions later (the off-diagonals of X) is still in that
+// XCD's L2: the stripe's B footprint is RR x n elements.
+template <typename T, int RR>
+__global__ void __launch_bounds__(64) k_csc_right_rr(int64_t rows, int64_t n, const int32_t *__restrict__ colptr,
+                                                     const int32_t *__restrict__ rowind, const T *__restrict__ val,
+                                                     const T *__restrict__ B, T *__restrict__ R, double2 alpha_,
+                                                     double2 beta_, int beta_zero, uint32_t ncg, int cc, int remap) {
+    constexpr int CW = 64 / RR;  // columns per step
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
+    const uint32_t cg = lb % ncg, rb = lb / ncg;
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)rb * RR + (lane % RR);
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+    const bool ok = i < rows;
+    const int64_t c0 = (int64_t)cg * cc * CW + lane / RR;
+    for (int q = 0; q < cc; ++q) {
+        const int64_t c = c0 + (int64_t)q * CW;
+        if (!ok || c >= n) continue;
+        const int32_t p0 = colptr[c], p1 = colptr[c + 1];
+        T acc = czero<T>();
+        for (int32_t p = p0; p < p1; ++p) acc = cfma(val[p], B[i + (int64_t)rowind[p] * rows], acc);
+        T *o = R + i + c * rows;
+        st_out(o, axpby(alpha, acc, beta, o, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
+    }
+}
+
 // ---- right with few rows (Bra, thin B): lanes along output columns.
 template <typename T>
 __global__ void __launch_bounds__(256) k_csc_right_thin(int64_t rows, int64_t n, const int32_t *__restrict__ colptr,
@@ -191,6 +221,7 @@ int g_sell_jchunk = 8;
 int g_csc_rb = 64;
 int g_csc_cc = 2;
 int g_csc_remap = 1;
+int g_csc_rows = 64;  // 64: one column per wave step (k_csc_right); 16 or 32: short stripes
 int g_nt_store = 1;
 
 template <typename T>
@@ -256,6 +287,25 @@ hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_
         else
             hipLaunchKernelGGL(k_csc_right_thin<float2>, dim3(g), dim3(256), 0, s, rows, X.n, X.colptr, X.row,
                                (const float2 *)X.val, (const float2 *)B, (float2 *)R, alpha, beta, bz);
+        return hipGetLastError();
+    }
+    if (g_csc_rows == 16 || g_csc_rows == 32) {
+        const int rr = g_csc_rows, cw = 64 / rr, cc = g_csc_cc;
+        const uint64_t nrb = (uint64_t)((rows + rr - 1) / rr);
+        const uint64_t ncg = (uint64_t)((X.n + (int64_t)cc * cw - 1) / ((int64_t)cc * cw));
+        const uint64_t G = nrb * ncg;
+        if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
+#define QOB_CRR(TT, RRV)                                                                                        \
+    hipLaunchKernelGGL((k_csc_right_rr<TT, RRV>), dim3((uint32_t)G), dim3(64), 0, s, rows, X.n, X.colptr, X.row, \
+                       (const TT *)X.val, (const TT *)B, (TT *)R, alpha, beta, bz, (uint32_t)ncg, cc, g_csc_remap)
+        if (dt == QOB_C64) {
+            if (rr == 16) QOB_CRR(double2, 16);
+            else QOB_CRR(double2, 32);
+        } else {
+            if (rr == 16) QOB_CRR(float2, 16);
+            else QOB_CRR(float2, 32);
+        }
+#undef QOB_CRR
         return hipGetLastError();
     }
     const int rb = g_csc_rb;

@@ -130,6 +130,10 @@ struct GatherPlanDev {
 // All launchers return hipSuccess or the launch error; they never allocate.
 
 // elementwise (k_blas1.hip)
+// Plan values from device-held coefficients (see k_refresh).
+hipError_t launch_refresh(qob_dtype dt, int nslots, int nvals, const int32_t *ptr, const int32_t *term,
+                          const double2 *bval, const int32_t *dst, const double2 *coef, void *vals, int32_t *prog,
+                          hipStream_t s);
 // Compute units of the current device, queried once per device.
 int device_cu_count();
 hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t s);

@@ -12,5 +12,6 @@ from .core import (AbstractOperator, Basis, Bra, CompositeBasis, Context, DenseO
 from .spin import sigmam, sigmap, sigmax, sigmay, sigmaz
 from .fock import create, destroy, number
 from .batch import Batch, Graph
+from .timedependent import TimeDependentSum
 
 __all__ = [n for n in dir() if not n.startswith("_")]

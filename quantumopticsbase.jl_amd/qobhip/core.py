@@ -514,6 +514,13 @@ class LazySum(AbstractOperator):
             fac = (qob_c64 * max(1, len(factors)))(*[c64(f) for f in factors])
             call("qob_lazysum_set_factors", context().h, self._h.h, fac)
 
+    def set_factors_device(self, dfactors, stream=None) -> None:
+        """Take the coefficients from device memory (len(operators) complex
+        doubles at ``dfactors``, a DeviceArray or a raw pointer), stream-ordered
+        and capturable; the host copy ``factors`` is not updated."""
+        ptr = dfactors.ptr if isinstance(dfactors, DeviceArray) else dfactors
+        call("qob_lazysum_set_factors_d", context().h, self._handle(), ptr, stream)
+
 
 class _Adjoint(AbstractOperator):
     """dagger of an operator with a device handle."""

@@ -8,8 +8,8 @@ of libqob_hip.so (qob_tune_set / qob_tune_get):
   qubit sums        qubit_tbits (tile = 2^tbits states), qubit_nth,
                     qubit_grid (persistent workgroups; 0 = 5 per CU),
                     qubit_remap (XCD-contiguous tiles), qubit_split
-  sparse products   sell_variant, sell_cj, sell_jchunk, csc_rb, csc_cc,
-                    csc_remap, nt_store
+  sparse products   sell_variant, sell_cj, sell_jchunk, csc_rows, csc_rb,
+                    csc_cc, csc_remap, nt_store
 
 ``tune`` runs a coordinate search: starting from the current values it
 tries every value of one tunable with the others fixed, keeps the fastest,
@@ -38,7 +38,7 @@ SPACES: Dict[str, Dict[str, List[int]]] = {
     "qubit": {"qubit_tbits": [10, 11, 12, 13], "qubit_nth": [256, 512, 1024], "qubit_grid": [0, 1024, 2048, 4096],
               "qubit_remap": [0, 1], "qubit_split": [0, 1]},
     "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16]},
-    "sparse_right": {"csc_rb": [64, 128, 256], "csc_cc": [1, 2, 4], "csc_remap": [0, 1]},
+    "sparse_right": {"csc_rows": [16, 32, 64], "csc_rb": [64, 128, 256], "csc_cc": [1, 2, 4, 8], "csc_remap": [0, 1]},
 }
 
 

@@ -329,3 +329,43 @@ def test_fused_equals_chain(dtype):
             _tune("fused_lt", 1)
     tol = TOL64 if dtype == np.complex128 else TOL32
     assert rel(outs[0], outs[1]) < tol
+
+
+@pytest.mark.parametrize("rows_tunable", [16, 32, 64])
+@pytest.mark.parametrize("cc", [1, 3])
+@pytest.mark.parametrize("adj", [False, True])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_sparse_right_stripes(rows_tunable, cc, adj, dtype, tol):
+    """rho * SparseOperator through every stripe height of the CSC right kernel
+    (rows >= 32, so the wide kernels run), beta = 0 over NaN and beta != 0."""
+    from qobhip import _lib
+    import ctypes as C
+    g = np.random.default_rng(31 + rows_tunable + cc)
+    m, n, rows = 150, 97, 77
+    M = sp.random(m, n, density=0.05, random_state=int(g.integers(1 << 30)), format="csc")
+    M = M + 1j * sp.random(m, n, density=0.05, random_state=int(g.integers(1 << 30)), format="csc")
+    oop = O.OSparse.from_scipy(sp.csc_matrix(M))
+    if adj:
+        oop = O.dagger(oop)
+    qop = to_q(oop)
+    mo, no = oop.shape
+    B = g.standard_normal((rows, mo)) + 1j * g.standard_normal((rows, mo))
+    saved = {}
+    for k in ("csc_rows", "csc_cc"):
+        v = C.c_int64()
+        _lib.call("qob_tune_get", k.encode(), C.byref(v))
+        saved[k] = v.value
+    try:
+        _lib.call("qob_tune_set", b"csc_rows", rows_tunable)
+        _lib.call("qob_tune_set", b"csc_cc", cc)
+        for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
+            R0 = np.full((rows, no), np.nan, complex) if beta == 0 else g.standard_normal((rows, no)) + 0j
+            want = np.zeros((rows, no), complex) if beta == 0 else R0.copy()
+            O.mul_op_right(want, B, oop, alpha, beta)
+            bl, br = gb(rows), gb(mo)
+            R = Q.Operator(bl, gb(no), R0, dtype)
+            Q.mul_(R, Q.Operator(bl, br, B, dtype), qop, alpha, beta)
+            assert rel(R.to_numpy(), want) < tol
+    finally:
+        for k, v in saved.items():
+            _lib.call("qob_tune_set", k.encode(), v)
