@@ -270,11 +270,21 @@ qob_status qob_normalize_ket(qob_ctx ctx, qob_dtype dt, int64_t n, void *x, qob_
 qob_status qob_normalize_op(qob_ctx ctx, qob_dtype dt, int64_t n, void *A, qob_stream s);
 /* expect(op, ket) = <psi|op|psi> (src/operators_dense.jl:219-221,
  * src/operators.jl:119) and expect(op, rho) = tr(op * rho)
- * (src/operators_dense.jl:223-230, src/operators_sparse.jl:36-45). */
+ * (src/operators_dense.jl:223-230, src/operators_sparse.jl:36-45).  In the
+ * fused mode these do not form op * state: a qubit sum or an all-sparse lazy
+ * operator reduces in the apply's own pass over a Ket, a SparseOperator
+ * reads only the entries of rho its nonzeros meet, an all-sparse lazy
+ * operator only the entries its diagonal terms meet. */
 qob_status qob_expect_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *psi,
                           int64_t n, qob_c64 *out, qob_stream s);
 qob_status qob_expect_op(qob_ctx ctx, qob_dtype dt, qob_op a, const void *rho,
                          int64_t n, qob_c64 *out, qob_stream s);
+/* variance(op, ket) = <psi|op op|psi> - <psi|op|psi>^2 with x = op psi
+ * computed once (src/operators.jl:139-143); FP64 result for both element
+ * types.  In the fused mode <psi|op x> is a single pass when op is a qubit
+ * sum or an all-sparse lazy operator. */
+qob_status qob_variance_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *psi,
+                            int64_t n, qob_c64 *out, qob_stream s);
 qob_status qob_dot_d(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x,
                      const void *y, void *dout, qob_stream s);
 qob_status qob_norm_d(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x,

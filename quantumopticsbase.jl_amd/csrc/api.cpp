@@ -780,7 +780,8 @@ struct QubitProgHost {
 
 // tb: tile bits of the launch; states longer than a tile (and at most 2 tb
 // bits) get their high-bit flips in a separate program.
-static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitProgHost &Q, int tb) {
+static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitProgHost &Q, int tb,
+                          int split_flag = -1) {
     struct QT {
         cd coef;
         uint32_t F;
@@ -827,7 +828,7 @@ static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitPr
         std::map<int, std::vector<cd>> win;
     };
     std::vector<GroupOut> outs[2];  // [0]: first pass, [1]: high-bit pass
-    const bool split = g_qubit_split && nbits > tb && nbits <= 2 * tb;
+    const bool split = (split_flag < 0 ? g_qubit_split : split_flag) && nbits > tb && nbits <= 2 * tb;
     for (auto &gq : groups) {
         GroupOut go;
         go.F = gq.first;
@@ -983,11 +984,13 @@ static bool qubit_terms(OpNode &op, bool trans, std::vector<TermSpec> &specs, st
 int g_qubit_enable = 1;
 extern int g_qubit_tbits;
 
-static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s, std::shared_ptr<QubitPlan> &out) {
+static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s, std::shared_ptr<QubitPlan> &out,
+                             bool nosplit = false) {
     std::lock_guard<std::mutex> g(op.mu);
     const int tb = g_qubit_tbits;
+    const int split = nosplit ? 0 : g_qubit_split;
     const std::string k = "qb:" + std::to_string(op.version) + (trans ? ":T:" : ":N:") + std::to_string((int)dt) +
-                          ":" + std::to_string(tb) + ":" + std::to_string(g_qubit_split);
+                          ":" + std::to_string(tb) + ":" + std::to_string(split);
     auto it = op.qplans.find(k);
     if (it == op.qplans.end()) {
         auto P = std::make_shared<QubitPlan>();
@@ -996,7 +999,7 @@ static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s
         std::vector<cd> base;
         int nbits = 0;
         QubitProgHost H;
-        if (qubit_terms(op, trans, specs, store, nbits, &base) && compile_qubit(nbits, specs, H, tb)) {
+        if (qubit_terms(op, trans, specs, store, nbits, &base) && compile_qubit(nbits, specs, H, tb, split)) {
             std::vector<int32_t> all = H.prog;
             if (!H.prog_hi.empty()) {
                 P->hi_off = (int32_t)all.size();
@@ -1020,7 +1023,7 @@ static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s
                     std::vector<TermSpec> unit = specs;
                     for (size_t q = 0; q < unit.size(); ++q) unit[q].coef = (q == t) ? base[q] : cd(0, 0);
                     QubitProgHost U;
-                    same = compile_qubit(nbits, unit, U, tb) && U.prog.size() == H.prog.size() &&
+                    same = compile_qubit(nbits, unit, U, tb, split) && U.prog.size() == H.prog.size() &&
                            U.prog_hi.size() == H.prog_hi.size() && U.vals.size() == H.vals.size();
                     if (!same) break;
                     std::vector<int32_t> ua = U.prog;
@@ -2092,11 +2095,117 @@ qob_status qob_normalize_op(qob_ctx ctx, qob_dtype dt, int64_t n, void *A, qob_s
     return QOB_OK;
 }
 
+// sum_J conj(u[J]) (op v)[J] without writing op v, when op has a fused
+// single-pass form: a qubit program or a square all-sparse gather plan.
+// done = false: no fused form, nothing launched.
+static qob_status fused_bilinear(qob_ctx ctx, hipStream_t st, qob_dtype dt, OpNode &op, const void *u,
+                                 const void *v, int64_t n, double2 *dout, bool &done) {
+    done = false;
+    if (op.kind != K_LS && op.kind != K_LT) return QOB_OK;
+    if (op.kind == K_LS && op.terms.empty()) return QOB_OK;
+    void *p;
+    if (g_qubit_enable) {
+        std::shared_ptr<QubitPlan> P;
+        QOB_TRY(qubit_plan(op, false, dt, st, P, true));
+        if (P->ok && P->hi_off < 0) {
+            const int tb = std::min(P->tbits, P->nbits);
+            const int nb = qubit_dot_blocks((int64_t)1 << (P->nbits - tb));
+            QOB_TRY(ws_get(ctx, st, WS_RED, (size_t)nb * sizeof(double2), p));
+            QubitStateDev S;
+            S.nbits = P->nbits;
+            S.prog_off = 0;
+            S.prog_hi_off = -1;
+            S.in = v;
+            S.out = nullptr;
+            hipError_t e = launch_qubit_dot(dt, S, u, P->tbits, (const int32_t *)P->prog.p, P->vals.p,
+                                            (double2 *)p, st);
+            if (e == hipSuccess) e = launch_reduce_partials(nb, (const double2 *)p, dout, st);
+            if (e != hipSuccess) return fail(QOB_EHIP, "fused expect: %s", hipGetErrorString(e));
+            done = true;
+            return QOB_OK;
+        }
+    }
+    GatherPlanDev G;
+    if (op.kind == K_LT) {
+        if (!lt_pure_sparse(op) || op.shl != op.shr) return QOB_OK;
+        std::vector<OpNode *> one{&op};
+        QOB_TRY(lazy_plan(op, "lt", one, {cd(1, 0)}, false, dt, G));
+    } else {
+        std::vector<OpNode *> lts;
+        for (auto &t : op.terms) {
+            if (!lt_pure_sparse(*t) || t->shl != t->shr) return QOB_OK;
+            lts.push_back(t.get());
+        }
+        QOB_TRY(lazy_plan(op, "ls" + std::to_string(op.version), lts, op.coef, false, dt, G, true, st));
+    }
+    const int nb = gather_dot_blocks(n);
+    QOB_TRY(ws_get(ctx, st, WS_RED, (size_t)nb * sizeof(double2), p));
+    hipError_t e = launch_gather_dot(dt, G, 0, u, v, n, (double2 *)p, st);
+    if (e == hipSuccess) e = launch_reduce_partials(nb, (const double2 *)p, dout, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "fused expect: %s", hipGetErrorString(e));
+    done = true;
+    return QOB_OK;
+}
+
+// tr(op V) for an n x n V without forming op V, when op is a SparseOperator
+// (one pass over its nonzeros, src/operators_sparse.jl:36-45) or a square
+// all-sparse LazyTensor / LazySum (one gathered element per diagonal term).
+static qob_status fused_trace(qob_ctx ctx, hipStream_t st, qob_dtype dt, OpNode &op, const void *V, int64_t n,
+                              double2 *dout, bool &done) {
+    done = false;
+    void *p;
+    if (op.kind == K_SPARSE) {
+        CscDev C;
+        QOB_TRY(sparse_csc(*op.sp, op.var, dt, C));
+        const int nb = csc_trace_blocks(n);
+        QOB_TRY(ws_get(ctx, st, WS_RED, (size_t)nb * sizeof(double2), p));
+        hipError_t e = launch_csc_trace(dt, C, V, n, (double2 *)p, st);
+        if (e == hipSuccess) e = launch_reduce_partials(nb, (const double2 *)p, dout, st);
+        if (e != hipSuccess) return fail(QOB_EHIP, "fused trace: %s", hipGetErrorString(e));
+        done = true;
+        return QOB_OK;
+    }
+    GatherPlanDev G;
+    if (op.kind == K_LT) {
+        if (!lt_pure_sparse(op) || op.shl != op.shr) return QOB_OK;
+        std::vector<OpNode *> one{&op};
+        QOB_TRY(lazy_plan(op, "lt", one, {cd(1, 0)}, false, dt, G));
+    } else if (op.kind == K_LS && !op.terms.empty()) {
+        std::vector<OpNode *> lts;
+        for (auto &t : op.terms) {
+            if (!lt_pure_sparse(*t) || t->shl != t->shr) return QOB_OK;
+            lts.push_back(t.get());
+        }
+        QOB_TRY(lazy_plan(op, "ls" + std::to_string(op.version), lts, op.coef, false, dt, G, true, st));
+    } else {
+        return QOB_OK;
+    }
+    const int nb = gather_dot_blocks(n);
+    QOB_TRY(ws_get(ctx, st, WS_RED, (size_t)nb * sizeof(double2), p));
+    hipError_t e = launch_gather_dot(dt, G, 1, nullptr, V, n, (double2 *)p, st);
+    if (e == hipSuccess) e = launch_reduce_partials(nb, (const double2 *)p, dout, st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "fused trace: %s", hipGetErrorString(e));
+    done = true;
+    return QOB_OK;
+}
+
+static qob_status scalar_to_host(qob_ctx ctx, hipStream_t st, const void *d, void *host, size_t bytes) {
+    (void)ctx;
+    QOB_HIP(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, st));
+    QOB_HIP(hipStreamSynchronize(st));
+    return QOB_OK;
+}
+
 qob_status qob_expect_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *psi, int64_t n, qob_c64 *out, qob_stream s) {
     if (!ctx || !a || !out) return fail(QOB_EARG, "null argument");
     OpNode &op = *a->node;
     if (op.m != n || op.n != n) return fail(QOB_EDIM, "expect: operator %lldx%lld on state %lld", (long long)op.m, (long long)op.n, (long long)n);
     hipStream_t st = strm(ctx, s);
+    void *d;
+    QOB_TRY(scal_ws(ctx, st, d));
+    bool done = false;
+    if (ctx->mode == QOB_MODE_FUSED) QOB_TRY(fused_bilinear(ctx, st, dt, op, psi, psi, n, (double2 *)d, done));
+    if (done) return scalar_to_host(ctx, st, d, out, 16);
     // tmp = op * psi; <psi|tmp>  (src/operators.jl:119)
     void *tmp;
     QOB_TRY(ws_get(ctx, st, WS_EXP, (size_t)n * esize(dt), tmp));
@@ -2110,12 +2219,56 @@ qob_status qob_expect_op(qob_ctx ctx, qob_dtype dt, qob_op a, const void *rho, i
     OpNode &op = *a->node;
     if (op.m != n || op.n != n) return fail(QOB_EDIM, "expect: operator %lldx%lld on state %lldx%lld", (long long)op.m, (long long)op.n, (long long)n, (long long)n);
     hipStream_t st = strm(ctx, s);
+    void *d;
+    QOB_TRY(scal_ws(ctx, st, d));
+    bool done = false;
+    if (ctx->mode == QOB_MODE_FUSED) QOB_TRY(fused_trace(ctx, st, dt, op, rho, n, (double2 *)d, done));
+    if (done) return scalar_to_host(ctx, st, d, out, 16);
     // tr(op * rho) (src/operators_dense.jl:223-230)
     void *tmp;
     QOB_TRY(ws_get(ctx, st, WS_EXP, (size_t)n * n * esize(dt), tmp));
     Apply ap{ctx, st, dt, 1, n, false};
     QOB_TRY(apply_op(ap, op, rho, tmp, cd(1, 0), cd(0, 0)));
     return qob_trace(ctx, dt, n, tmp, out, s);
+}
+
+qob_status qob_variance_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *psi, int64_t n, qob_c64 *out,
+                            qob_stream s) {
+    if (!ctx || !a || !out) return fail(QOB_EARG, "null argument");
+    OpNode &op = *a->node;
+    if (op.m != n || op.n != n) return fail(QOB_EDIM, "variance: operator %lldx%lld on state %lld", (long long)op.m, (long long)op.n, (long long)n);
+    hipStream_t st = strm(ctx, s);
+    // x = op psi; psi' (op x) - (psi' x)^2  (src/operators.jl:139-143)
+    void *x, *d;
+    QOB_TRY(ws_get(ctx, st, WS_EXP, (size_t)n * esize(dt), x));
+    QOB_TRY(scal_ws(ctx, st, d));
+    Apply ap{ctx, st, dt, 1, 1, false};
+    QOB_TRY(apply_op(ap, op, psi, x, cd(1, 0), cd(0, 0)));
+    double2 *dd = (double2 *)d;
+    {
+        double2 *part;
+        QOB_TRY(red_ws(ctx, st, n, part));
+        hipError_t e = launch_dot(dt, n, psi, x, part, dd, st);
+        if (e != hipSuccess) return fail(QOB_EHIP, "variance: %s", hipGetErrorString(e));
+    }
+    bool done = false;
+    if (ctx->mode == QOB_MODE_FUSED) QOB_TRY(fused_bilinear(ctx, st, dt, op, psi, x, n, dd + 1, done));
+    if (!done) {
+        void *y;
+        QOB_TRY(ws_get(ctx, st, WS_TMP1, (size_t)n * esize(dt), y));
+        QOB_TRY(apply_op(ap, op, x, y, cd(1, 0), cd(0, 0)));
+        double2 *part;
+        QOB_TRY(red_ws(ctx, st, n, part));
+        hipError_t e = launch_dot(dt, n, psi, y, part, dd + 1, st);
+        if (e != hipSuccess) return fail(QOB_EHIP, "variance: %s", hipGetErrorString(e));
+    }
+    double2 h[2];
+    QOB_TRY(scalar_to_host(ctx, st, d, h, sizeof(h)));
+    const cd ex(h[0].x, h[0].y), ex2(h[1].x, h[1].y);
+    const cd v = ex2 - ex * ex;
+    out->re = v.real();
+    out->im = v.imag();
+    return QOB_OK;
 }
 
 }  // extern "C"

@@ -325,6 +325,11 @@ static hipError_t reduce_any(qob_dtype dt, int64_t n, const void *x, const void 
     return hipGetLastError();
 }
 
+hipError_t launch_reduce_partials(int nparts, const double2 *partials, double2 *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce2<0>, 1, BS, 0, s, nparts, partials, (void *)out);
+    return hipGetLastError();
+}
+
 hipError_t launch_dot(qob_dtype dt, int64_t n, const void *x, const void *y, double2 *partials, double2 *out, hipStream_t s) {
     return reduce_any<0>(dt, n, x, y, 1, partials, out, s);
 }

@@ -21,6 +21,8 @@
 //                the whole term walk is wave-uniform.
 #include "qob_internal.h"
 
+#include <algorithm>
+
 namespace qob {
 
 namespace {
@@ -147,7 +149,59 @@ __global__ void __launch_bounds__(256) k_gather_cols(GatherPlanDev P, const T *_
     }
 }
 
+// FP64 block sum of a double2 (256 threads); lane 0 of wave 0 gets the total
+__device__ __forceinline__ double2 block_sum256(double2 v) {
+    __shared__ double2 sh[256];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[threadIdx.x].x += sh[threadIdx.x + w].x;
+            sh[threadIdx.x].y += sh[threadIdx.x + w].y;
+        }
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_gather_dot(GatherPlanDev P, const T *__restrict__ u, const T *__restrict__ v,
+                                                    int64_t n, double2 *__restrict__ partials) {
+    double2 r = make_double2(0, 0);
+    for (int64_t J = blockIdx.x * (int64_t)256 + threadIdx.x; J < n; J += (int64_t)gridDim.x * 256) {
+        Digits dg{0, 0};
+        T acc;
+        if (MODE == 0) {
+            acc = eval_terms<T, true>((uint32_t)J, P, dg, [&](int64_t K) { return v[K]; });
+            const T a = u[J];
+            r.x += (double)a.x * acc.x + (double)a.y * acc.y;
+            r.y += (double)a.x * acc.y - (double)a.y * acc.x;
+        } else {
+            const T *col = v + J * n;
+            acc = eval_terms<T, true>((uint32_t)J, P, dg, [&](int64_t K) { return col[K]; });
+            r.x += acc.x;
+            r.y += acc.y;
+        }
+    }
+    const double2 t = block_sum256(r);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 }  // namespace
+
+int gather_dot_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256)); }
+
+hipError_t launch_gather_dot(qob_dtype dt, const GatherPlanDev &P, int mode, const void *u, const void *v, int64_t n,
+                             double2 *partials, hipStream_t s) {
+    if (!P.square) return hipErrorInvalidValue;
+    const int g = gather_dot_blocks(n);
+#define QOB_GD(TT, MD) \
+    hipLaunchKernelGGL((k_gather_dot<TT, MD>), dim3(g), dim3(256), 0, s, P, (const TT *)u, (const TT *)v, n, partials)
+    if (dt == QOB_C64) { if (mode == 0) QOB_GD(double2, 0); else QOB_GD(double2, 1); }
+    else { if (mode == 0) QOB_GD(float2, 0); else QOB_GD(float2, 1); }
+#undef QOB_GD
+    return hipGetLastError();
+}
 
 hipError_t launch_gather_rows(qob_dtype dt, const GatherPlanDev &P, const void *in, int64_t n_in, void *out,
                               int64_t n_out, int64_t nbatch, double2 alpha, double2 beta, hipStream_t s) {

@@ -30,6 +30,9 @@
 // depend on group g and is loaded while group g is applied.
 #include "qob_internal.h"
 
+#include <algorithm>
+#include <atomic>
+
 namespace qob {
 
 namespace {
@@ -304,6 +307,64 @@ __global__ void __launch_bounds__(NTH) k_qubit_hi(const QubitStateDev *__restric
     }
 }
 
+// sum_J conj(u[J]) (H v)[J] for one state (expect / variance): the same tile
+// walk as k_qubit with the store replaced by an FP64 reduction; one partial
+// per workgroup (fixed item order, so the result is deterministic).  High
+// flips read partner tiles from v in global memory.
+template <typename T, int NTH, int TBITS>
+__global__ void __launch_bounds__(NTH) k_qubit_dot(QubitStateDev S, const T *__restrict__ u, int ntiles,
+                                                   const int32_t *__restrict__ prog, const T *__restrict__ vals,
+                                                   double2 *__restrict__ partials) {
+    constexpr int EPT = (1 << TBITS) / NTH;
+    extern __shared__ unsigned char lds_raw[];
+    T *tile = (T *)lds_raw;
+    __shared__ double2 red[NTH / 64];
+    const int tid = threadIdx.x;
+    const int tb = S.nbits < TBITS ? S.nbits : TBITS;
+    const int tsize = 1 << tb;
+    const T *__restrict__ v = (const T *)S.in;
+    double rx = 0, ry = 0;
+    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
+        const uint32_t base = (uint32_t)it << tb;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NTH;
+            if (e < tsize) tile[e] = v[base + e];
+        }
+        __syncthreads();
+        T acc[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
+        const LoMap m{base, (uint32_t)tsize - 1u, tb};
+        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, v, m, acc);
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NTH;
+            if (e < tsize) {
+                const T a = u[base + e];
+                rx += (double)a.x * acc[k].x + (double)a.y * acc[k].y;
+                ry += (double)a.x * acc[k].y - (double)a.y * acc[k].x;
+            }
+        }
+    }
+    // wave sums, then one lane adds the waves in a fixed order
+    for (int off = 32; off > 0; off >>= 1) {
+        rx += __shfl_down(rx, off, 64);
+        ry += __shfl_down(ry, off, 64);
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = make_double2(rx, ry);
+    __syncthreads();
+    if (tid == 0) {
+        double2 t = make_double2(0, 0);
+        for (int w = 0; w < NTH / 64; ++w) {
+            t.x += red[w].x;
+            t.y += red[w].y;
+        }
+        partials[blockIdx.x] = t;
+    }
+}
+
 // Elementwise batch update out += a * in over the same items (the axpy of a
 // time step for every state of a batch).
 template <typename T, int NTH>
@@ -457,6 +518,46 @@ hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncol
     if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
     return launch_qubit_hi_any(dt, nullptr, nullptr, (int)nitems, tbits, prog, vals, alpha, beta, s, single,
                                (int)ntiles);
+}
+
+int qubit_dot_blocks(int64_t ntiles) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)device_cu_count() * 2));
+}
+
+hipError_t launch_qubit_dot(qob_dtype dt, QubitStateDev single, const void *u, int tbits, const int32_t *prog,
+                            const void *vals, double2 *partials, hipStream_t s) {
+    if (tbits < 10 || tbits > 13) return hipErrorInvalidValue;
+    const int tb = std::min(tbits, (int)single.nbits);
+    const int64_t ntiles = (int64_t)1 << (single.nbits - tb);
+    if (ntiles >= (1ll << 31)) return hipErrorInvalidValue;
+    const int g = qubit_dot_blocks(ntiles);
+    const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
+#define QOB_QD(TT, TB)                                                                                          \
+    do {                                                                                                        \
+        auto kfn = k_qubit_dot<TT, 512, TB>;                                                                    \
+        static std::atomic<int> lds_set{0};                                                                     \
+        if (!lds_set.load()) {                                                                                  \
+            hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                                150 * 1024);                                                    \
+            if (e_ != hipSuccess) return e_;                                                                    \
+            lds_set.store(1);                                                                                   \
+        }                                                                                                       \
+        hipLaunchKernelGGL(kfn, dim3(g), dim3(512), lds, s, single, (const TT *)u, (int)ntiles, prog,           \
+                           (const TT *)vals, partials);                                                         \
+        return hipGetLastError();                                                                               \
+    } while (0)
+    if (dt == QOB_C64) {
+        if (tbits == 10) QOB_QD(double2, 10);
+        else if (tbits == 11) QOB_QD(double2, 11);
+        else if (tbits == 12) QOB_QD(double2, 12);
+        else QOB_QD(double2, 13);
+    } else {
+        if (tbits == 10) QOB_QD(float2, 10);
+        else if (tbits == 11) QOB_QD(float2, 11);
+        else if (tbits == 12) QOB_QD(float2, 12);
+        else QOB_QD(float2, 13);
+    }
+#undef QOB_QD
 }
 
 hipError_t launch_qubit_axpy(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,

@@ -18,6 +18,8 @@
 // tables built at upload time.
 #include "qob_internal.h"
 
+#include <algorithm>
+
 namespace qob {
 
 namespace {
@@ -212,7 +214,48 @@ __global__ void __launch_bounds__(256) k_csc_right_thin(int64_t rows, int64_t n,
     }
 }
 
+// tr(X V) = sum over nonzeros (row r, column c) of X of val * V[c, r]
+// (expect(::SparseOpPureType, ::Operator), src/operators_sparse.jl:36-45):
+// one pass over the nonzeros, one element of V each.
+template <typename T>
+__global__ void __launch_bounds__(256) k_csc_trace(int64_t n, const int32_t *__restrict__ colptr,
+                                                   const int32_t *__restrict__ rowind, const T *__restrict__ val,
+                                                   const T *__restrict__ V, double2 *__restrict__ partials) {
+    __shared__ double2 sh[256];
+    double2 r = make_double2(0, 0);
+    for (int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x; c < n; c += (int64_t)gridDim.x * 256) {
+        for (int32_t p = colptr[c]; p < colptr[c + 1]; ++p) {
+            const T a = val[p], b = V[c + (int64_t)rowind[p] * n];
+            r.x += (double)a.x * b.x - (double)a.y * b.y;
+            r.y += (double)a.x * b.y + (double)a.y * b.x;
+        }
+    }
+    sh[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[threadIdx.x].x += sh[threadIdx.x + w].x;
+            sh[threadIdx.x].y += sh[threadIdx.x + w].y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
 }  // namespace
+
+int csc_trace_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256)); }
+
+hipError_t launch_csc_trace(qob_dtype dt, const CscDev &X, const void *V, int64_t n, double2 *partials, hipStream_t s) {
+    const int g = csc_trace_blocks(n);
+    if (dt == QOB_C64)
+        hipLaunchKernelGGL(k_csc_trace<double2>, dim3(g), dim3(256), 0, s, n, X.colptr, X.row, (const double2 *)X.val,
+                           (const double2 *)V, partials);
+    else
+        hipLaunchKernelGGL(k_csc_trace<float2>, dim3(g), dim3(256), 0, s, n, X.colptr, X.row, (const float2 *)X.val,
+                           (const float2 *)V, partials);
+    return hipGetLastError();
+}
 
 // Tunables (set once from the host; defaults chosen from measurements).
 int g_sell_variant = 1;  // 0: column groups of CJ, 1: register-resident rows

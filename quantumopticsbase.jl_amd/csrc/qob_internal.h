@@ -134,6 +134,8 @@ struct GatherPlanDev {
 hipError_t launch_refresh(qob_dtype dt, int nslots, int nvals, const int32_t *ptr, const int32_t *term,
                           const double2 *bval, const int32_t *dst, const double2 *coef, void *vals, int32_t *prog,
                           hipStream_t s);
+// Final stage of a two-stage FP64 reduction: *out = sum of nparts partials.
+hipError_t launch_reduce_partials(int nparts, const double2 *partials, double2 *out, hipStream_t s);
 // Compute units of the current device, queried once per device.
 int device_cu_count();
 hipError_t launch_scale(qob_dtype dt, int64_t n, double2 a, void *x, hipStream_t s);
@@ -240,5 +242,21 @@ hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncol
 // out (D x ncols) = alpha * (A_n ... A_1) in + beta * out
 hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols,
                            double2 alpha, double2 beta, hipStream_t s);
+
+// Fused reductions over a square gather plan (FP64 partials, one per block):
+//   mode 0: sum_J conj(u[J]) (H v)[J]          (expect / bilinear on a Ket)
+//   mode 1: sum_J (H V[:, J])[J], V n x n       (tr(H V), expect on rho)
+int gather_dot_blocks(int64_t n);
+hipError_t launch_gather_dot(qob_dtype dt, const GatherPlanDev &P, int mode, const void *u, const void *v, int64_t n,
+                             double2 *partials, hipStream_t s);
+// sum_J conj(u[J]) (H v)[J] for one qubit state (single.in = v), FP64
+// partials, one per workgroup of the returned grid size
+int qubit_dot_blocks(int64_t ntiles);
+hipError_t launch_qubit_dot(qob_dtype dt, QubitStateDev single, const void *u, int tbits, const int32_t *prog,
+                            const void *vals, double2 *partials, hipStream_t s);
+// sum over the nonzeros of a CSC matrix X (n columns) of val * V[c + row * n]:
+// tr(X V) for an n x n V (expect(::SparseOpPureType, ::Operator))
+int csc_trace_blocks(int64_t n);
+hipError_t launch_csc_trace(qob_dtype dt, const CscDev &X, const void *V, int64_t n, double2 *partials, hipStream_t s);
 
 }  // namespace qob

@@ -8,7 +8,7 @@ from ._lib import (QOB_C32, QOB_C64, QOB_MODE_FUSED, QOB_MODE_PLAIN, ArgumentErr
 from .core import (AbstractOperator, Basis, Bra, CompositeBasis, Context, DenseOperator, DeviceArray, EyeOperator,
                    FockBasis, GenericBasis, Ket, LazySum, LazyTensor, NLevelBasis, Operator, SparseOperator,
                    SpinBasis, axpy_, context, dagger, dot, expect, identityoperator, mul_, norm, normalize_,
-                   synchronize, tensor, tr)
+                   synchronize, tensor, tr, variance)
 from .spin import sigmam, sigmap, sigmax, sigmay, sigmaz
 from .fock import create, destroy, number
 from .batch import Batch, Graph

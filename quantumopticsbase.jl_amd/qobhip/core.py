@@ -724,6 +724,20 @@ def expect(op: AbstractOperator, state) -> complex:
     return complex(out.re, out.im)
 
 
+def variance(op: AbstractOperator, state: "Ket") -> complex:
+    """variance(op, ket) = <psi|op op|psi> - <psi|op|psi>^2 with x = op psi
+    formed once (src/operators.jl:139-143)."""
+    if not isinstance(state, Ket):
+        raise MethodError(f"variance not defined for {type(state).__name__}")
+    if isinstance(op, Operator):
+        x = op * state
+        return dot(state, op * x) - dot(state, x) ** 2
+    out = qob_c64()
+    call("qob_variance_ket", context().h, _dt(state.dtype), op._handle(), state.data.ptr, state.data.n,
+         C.byref(out), None)
+    return complex(out.re, out.im)
+
+
 def axpy_(y: _State, a, x: _State) -> _State:
     """y .+= a .* x on device data."""
     _check_same_dtype(x, y)
