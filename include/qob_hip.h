@@ -137,6 +137,11 @@ qob_status qob_lazytensor_create(qob_ctx ctx, int nsites,
 qob_status qob_lazysum_create(qob_ctx ctx, int64_t m, int64_t n, int nterms,
                               const qob_c64 *factors, const qob_op *operators,
                               qob_op *out);
+/* LazyProduct(operators, factor) (src/operators_lazyproduct.jl:24-58):
+ * factor * A1 * A2 * ... * An, applied factor by factor through temporaries
+ * (mul! :103-163); operators[i-1] must have as many columns as operators[i]
+ * has rows (check_multiplicable, :4-10). */
+qob_status qob_lazyproduct_create(qob_ctx ctx, int nops, const qob_op *operators, qob_c64 factor, qob_op *out);
 /* Replace the coefficient vector of a LazySum in place (what
  * TimeDependentSum's set_time! does, src/time_dependent_operator.jl:162-170). */
 qob_status qob_lazysum_set_factors(qob_ctx ctx, qob_op sum, const qob_c64 *factors);

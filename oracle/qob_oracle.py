@@ -128,6 +128,17 @@ class OLazySum:
         return (self.m, self.n)
 
 
+@dataclass
+class OLazyProduct:
+    """LazyProduct (src/operators_lazyproduct.jl:24-58): factor * A1 ... An."""
+    operators: List
+    factor: complex = 1.0
+
+    @property
+    def shape(self):
+        return (self.operators[0].shape[0], self.operators[-1].shape[1])
+
+
 def dagger(op):
     """dagger (src/operators_dense.jl:128 / LazyTensor, LazySum algebra)."""
     if isinstance(op, OSparse):
@@ -141,6 +152,8 @@ def dagger(op):
                            np.conj(op.factor))
     if isinstance(op, OLazySum):
         return OLazySum(op.n, op.m, [np.conj(f) for f in op.factors], [dagger(o) for o in op.operators])
+    if isinstance(op, OLazyProduct):
+        return OLazyProduct([dagger(o) for o in reversed(op.operators)], np.conj(op.factor))
     raise TypeError(type(op))
 
 
@@ -175,6 +188,11 @@ def dense_of(op) -> np.ndarray:
         out = np.zeros((op.m, op.n), dtype=np.complex128)
         for f, o in zip(op.factors, op.operators):
             out = out + f * dense_of(o)
+        return out
+    if isinstance(op, OLazyProduct):
+        out = op.factor * dense_of(op.operators[0])
+        for o in op.operators[1:]:
+            out = out @ dense_of(o)
         return out
     return _mat(op)
 
@@ -631,8 +649,39 @@ def mul_lt_op_right(result, a, b: OLazyTensor, alpha, beta):
 
 
 # ------------------------------------------------------------ generic mul!
+
+# ------------------------------------------- src/operators_lazyproduct.jl restated
+def _lp_chain(result, lp, b, alpha, beta, mul, right):
+    """mul! of a LazyProduct (src/operators_lazyproduct.jl:103-163): the first
+    application takes the product's factor, the last (alpha, beta), the
+    ones between (1, 0).  right=False runs the last factor first (Ket,
+    Operator left); right=True the first factor first (Bra, Operator right)."""
+    if alpha == 0:
+        return _zero_op_mul(result, beta)
+    ops = lp.operators
+    if len(ops) == 1:
+        return mul(result, ops[0], b, lp.factor * alpha, beta)
+    order = list(ops) if right else list(reversed(ops))
+    tmp = b
+    for i, o in enumerate(order[:-1]):
+        out_shape = _apply_shape(o, tmp, right)
+        nxt = np.zeros(out_shape, dtype=np.complex128)
+        mul(nxt, o, tmp, lp.factor if i == 0 else 1.0, 0.0)
+        tmp = nxt
+    return mul(result, order[-1], tmp, alpha, beta)
+
+
+def _apply_shape(o, x, right):
+    m, n = o.shape
+    if x.ndim == 1:
+        return (n,) if right else (m,)
+    return (x.shape[0], n) if right else (m, x.shape[1])
+
+
 def mul_ket(result, a, b, alpha=1.0, beta=0.0):
     """mul!(result::Ket, a, b::Ket, alpha, beta)."""
+    if isinstance(a, OLazyProduct):
+        return _lp_chain(result, a, b, alpha, beta, mul_ket, False)
     if isinstance(a, OLazySum):
         # src/operators_lazysum.jl:189-200
         if len(a.operators) == 0 or alpha == 0:
@@ -651,6 +700,8 @@ def mul_ket(result, a, b, alpha=1.0, beta=0.0):
 
 def mul_bra(result, a, b, alpha=1.0, beta=0.0):
     """mul!(result::Bra, a::Bra, b, alpha, beta)."""
+    if isinstance(b, OLazyProduct):
+        return _lp_chain(result, b, a, alpha, beta, lambda r, o, x, al, be: mul_bra(r, x, o, al, be), True)
     if isinstance(b, OLazySum):
         # src/operators_lazysum.jl:202-213
         if len(b.operators) == 0 or alpha == 0:
@@ -669,6 +720,8 @@ def mul_bra(result, a, b, alpha=1.0, beta=0.0):
 
 def mul_op_left(result, a, b, alpha=1.0, beta=0.0):
     """mul!(result::Operator, a, b::Operator, alpha, beta)."""
+    if isinstance(a, OLazyProduct):
+        return _lp_chain(result, a, b, alpha, beta, mul_op_left, False)
     if isinstance(a, OLazySum):
         # src/operators_lazysum.jl:215-226
         if len(a.operators) == 0 or alpha == 0:
@@ -687,6 +740,8 @@ def mul_op_left(result, a, b, alpha=1.0, beta=0.0):
 
 def mul_op_right(result, a, b, alpha=1.0, beta=0.0):
     """mul!(result::Operator, a::Operator, b, alpha, beta)."""
+    if isinstance(b, OLazyProduct):
+        return _lp_chain(result, b, a, alpha, beta, lambda r, o, x, al, be: mul_op_right(r, x, o, al, be), True)
     if isinstance(b, OLazySum):
         # src/operators_lazysum.jl:227-238
         if len(b.operators) == 0 or alpha == 0:

@@ -326,7 +326,7 @@ static qob_status dense_dev(DenseData &D, int v, qob_dtype dt, const void *&ptr)
 }
 
 // ---------------------------------------------------------------- op nodes
-enum Kind { K_SPARSE, K_DENSE, K_EYE, K_LT, K_LS };
+enum Kind { K_SPARSE, K_DENSE, K_EYE, K_LT, K_LS, K_LP };
 
 // Linear map from a LazySum's coefficients to a plan's values: slot s (a
 // value of the plan's table when s < nvals, else a complex double inline in
@@ -432,8 +432,13 @@ struct qob_op_s {
 namespace qob {
 
 // ------------------------------------------------------------- workspace
+// slots: chain temporaries, reductions, scalars, expect temporary, and two
+// LazyProduct temporaries per nesting depth (up to 4)
+enum { WS_TMP1 = 0, WS_TMP2 = 1, WS_RED = 2, WS_SCAL = 3, WS_EXP = 4, WS_LP0 = 5, WS_NSLOT = 13 };
+constexpr int LP_MAX_DEPTH = 4;
+
 struct Workspace {
-    DevBuf slot[5];
+    DevBuf slot[WS_NSLOT];
 };
 
 }  // namespace qob
@@ -450,7 +455,6 @@ struct qob_ctx_s {
 
 namespace qob {
 
-enum { WS_TMP1 = 0, WS_TMP2 = 1, WS_RED = 2, WS_SCAL = 3, WS_EXP = 4, WS_NSLOT = 5 };
 
 static qob_status ws_get(qob_ctx ctx, hipStream_t s, int slot, size_t bytes, void *&p) {
     Workspace *w;
@@ -1112,6 +1116,7 @@ struct Apply {
     qob_dtype dt;
     int64_t lead, trail;  // one of them is 1
     bool trans;           // use transposed factors (right / Bra)
+    int depth = 0;        // LazyProduct nesting (selects its temporaries)
 };
 
 static qob_status apply_op(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta);
@@ -1464,8 +1469,36 @@ static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *ou
     return QOB_OK;
 }
 
+// LazyProduct (src/operators_lazyproduct.jl:103-163): the factors applied in
+// turn through two temporaries of the view (ket_l / bra_r / tmp1, tmp2 of the
+// reference), the product's factor on the first application and (alpha,
+// beta) on the last.  A Ket or a left application runs the last factor
+// first; a Bra or a right application (transposed) the first factor first.
+static qob_status apply_lp(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
+    const int64_t dout = op_out_dim(op, ap.trans);
+    if (is_zero(alpha)) return apply_scale(ap, dout, out, beta);
+    const size_t n = op.terms.size();
+    if (n == 1) return apply_op(ap, *op.terms[0], in, out, alpha * op.factor, beta);
+    if (ap.depth >= LP_MAX_DEPTH) return fail(QOB_EUNSUPPORTED, "LazyProduct nested more than %d deep", LP_MAX_DEPTH);
+    Apply sub = ap;
+    sub.depth = ap.depth + 1;
+    auto fac = [&](size_t k) -> OpNode & { return *op.terms[ap.trans ? k : n - 1 - k]; };
+    const void *src = in;
+    for (size_t k = 0; k + 1 < n; ++k) {
+        OpNode &f = fac(k);
+        const int64_t d = op_out_dim(f, ap.trans);
+        void *dst;
+        QOB_TRY(ws_get(ap.ctx, ap.s, WS_LP0 + 2 * ap.depth + (int)(k & 1),
+                       (size_t)(d * ap.lead * ap.trail) * esize(ap.dt), dst));
+        QOB_TRY(apply_op(sub, f, src, dst, k == 0 ? op.factor : cd(1, 0), cd(0, 0)));
+        src = dst;
+    }
+    return apply_op(sub, fac(n - 1), src, out, alpha, beta);
+}
+
 static qob_status apply_op(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
     switch (op.kind) {
+    case K_LP: return apply_lp(ap, op, in, out, alpha, beta);
     case K_SPARSE: return apply_sparse(ap, op, in, out, alpha, beta);
     case K_DENSE: return apply_dense(ap, op, in, out, alpha, beta);
     case K_EYE: return apply_embed(ap, op_out_dim(op, ap.trans), op_in_dim(op, ap.trans), in, out, alpha, beta);
@@ -1721,12 +1754,40 @@ static std::shared_ptr<OpNode> adjoint_node(const std::shared_ptr<OpNode> &a) {
         r->fsite = a->fsite;
         for (auto &f : a->fop) r->fop.push_back(adjoint_node(f));
         break;
-    case K_LS:
+    case K_LS: {
         for (auto &c : a->coef) r->coef.push_back(std::conj(c));
         for (auto &t : a->terms) r->terms.push_back(adjoint_node(t));
+        std::vector<double2> dc(r->coef.size());
+        for (size_t t = 0; t < dc.size(); ++t) dc[t] = d2(r->coef[t]);
+        if (!dc.empty()) (void)r->dcoef.upload(dc.data(), dc.size() * sizeof(double2));
+        break;
+    }
+    case K_LP:  // (A1 ... An)^dag = An^dag ... A1^dag
+        r->factor = std::conj(a->factor);
+        for (auto it = a->terms.rbegin(); it != a->terms.rend(); ++it) r->terms.push_back(adjoint_node(*it));
         break;
     }
     return r;
+}
+
+qob_status qob_lazyproduct_create(qob_ctx ctx, int nops, const qob_op *operators, qob_c64 factor, qob_op *out) {
+    if (!ctx || !out || !operators) return fail(QOB_EARG, "null argument");
+    if (nops < 1) return fail(QOB_EARG, "LazyProduct needs at least one operator!");
+    auto node = std::make_shared<OpNode>();
+    node->kind = K_LP;
+    node->factor = cdv(factor);
+    for (int i = 0; i < nops; ++i) {
+        if (!operators[i]) return fail(QOB_EARG, "null operator %d", i + 1);
+        auto &x = operators[i]->node;
+        if (i > 0 && node->terms.back()->n != x->m)
+            return fail(QOB_EBASIS, "LazyProduct: operator %d is %lldx%lld after one with %lld columns", i + 1,
+                        (long long)x->m, (long long)x->n, (long long)node->terms.back()->n);
+        node->terms.push_back(x);
+    }
+    node->m = node->terms.front()->m;
+    node->n = node->terms.back()->n;
+    *out = wrap(node);
+    return QOB_OK;
 }
 
 qob_status qob_op_adjoint(qob_ctx ctx, qob_op a, qob_op *out) {
@@ -1756,7 +1817,8 @@ qob_status qob_lazytensor_create(qob_ctx ctx, int nsites, const int64_t *shape_l
             return fail(QOB_EARG, "indices must be sorted and unique");  // src/operators_lazytensor.jl:26
         if (!factors[i]) return fail(QOB_EARG, "null factor");
         const OpNode &f = *factors[i]->node;
-        if (f.kind == K_LT || f.kind == K_LS) return fail(QOB_EUNSUPPORTED, "LazyTensor factors must be data operators");
+        if (f.kind == K_LT || f.kind == K_LS || f.kind == K_LP)
+            return fail(QOB_EUNSUPPORTED, "LazyTensor factors must be data operators");
         if (f.m != shape_l[s] || f.n != shape_r[s])
             return fail(QOB_EBASIS, "factor %d is %lldx%lld but site %lld is %lldx%lld", i + 1, (long long)f.m,
                         (long long)f.n, (long long)(s + 1), (long long)shape_l[s], (long long)shape_r[s]);
@@ -1828,6 +1890,7 @@ qob_status qob_lazysum_set_factors_d(qob_ctx ctx, qob_op sum, const void *dfacto
             return fail(QOB_EUNSUPPORTED, "device-held coefficients need every term to be an all-sparse LazyTensor");
     const hipStream_t st = strm(ctx, s);
     std::lock_guard<std::mutex> g(n.mu);
+    if (!n.dcoef.p) QOB_TRY(n.dcoef.alloc(n.terms.size() * sizeof(double2)));
     hipError_t e = launch_copy(QOB_C64, (int64_t)n.terms.size(), dfactors, n.dcoef.p, st);
     if (e != hipSuccess) return fail(QOB_EHIP, "coefficient copy: %s", hipGetErrorString(e));
     for (auto &kv : n.qplans) {

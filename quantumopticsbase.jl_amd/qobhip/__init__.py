@@ -6,7 +6,7 @@ Host-side mirror of the reference API over the C ABI of libqob_hip.so
 from ._lib import (QOB_C32, QOB_C64, QOB_MODE_FUSED, QOB_MODE_PLAIN, ArgumentError, DimensionMismatch,
                    IncompatibleBases, MethodError, QobError)
 from .core import (AbstractOperator, Basis, Bra, CompositeBasis, Context, DenseOperator, DeviceArray, EyeOperator,
-                   FockBasis, GenericBasis, Ket, LazySum, LazyTensor, NLevelBasis, Operator, SparseOperator,
+                   FockBasis, GenericBasis, Ket, LazyProduct, LazySum, LazyTensor, NLevelBasis, Operator, SparseOperator,
                    SpinBasis, axpy_, context, dagger, dot, expect, identityoperator, mul_, norm, normalize_,
                    synchronize, tensor, tr, variance)
 from .spin import sigmam, sigmap, sigmax, sigmay, sigmaz

@@ -93,6 +93,7 @@ _PROTOS = {
     "qob_op_adjoint": (_i32, [_vp, _vp, _P(_vp)]),
     "qob_lazytensor_create": (_i32, [_vp, _i32, _P(_i64), _P(_i64), _i32, _P(_i64), _P(_vp), qob_c64, _P(_vp)]),
     "qob_lazysum_create": (_i32, [_vp, _i64, _i64, _i32, _P(qob_c64), _P(_vp), _P(_vp)]),
+    "qob_lazyproduct_create": (_i32, [_vp, _i32, _P(_vp), qob_c64, _P(_vp)]),
     "qob_lazysum_set_factors": (_i32, [_vp, _vp, _P(qob_c64)]),
     "qob_lazysum_set_factors_d": (_i32, [_vp, _vp, _vp, _vp]),
     "qob_op_shape": (_i32, [_vp, _P(_i64), _P(_i64)]),

@@ -522,6 +522,39 @@ class LazySum(AbstractOperator):
         call("qob_lazysum_set_factors_d", context().h, self._handle(), ptr, stream)
 
 
+class LazyProduct(AbstractOperator):
+    """LazyProduct(operators[, factor]) or LazyProduct(op1, op2, ...)
+    (src/operators_lazyproduct.jl:24-58): factor * op1 * op2 * ... applied
+    factor by factor (mul! :103-163)."""
+
+    def __init__(self, *args, factor=1.0):
+        if len(args) == 1 and isinstance(args[0], (list, tuple)):
+            ops = tuple(args[0])
+        elif len(args) == 2 and isinstance(args[0], (list, tuple)):
+            ops, factor = tuple(args[0]), args[1]
+        else:
+            ops = tuple(args)
+        if not ops:
+            raise ArgumentError("LazyProduct needs at least one operator!")
+        for a, b in zip(ops[:-1], ops[1:]):
+            if a.basis_r != b.basis_l:
+                raise IncompatibleBases("LazyProduct: operators are not multiplicable")
+        self.operators = ops
+        self.factor = complex(factor)
+        self.basis_l, self.basis_r = ops[0].basis_l, ops[-1].basis_r
+        self._h = None
+
+    def _handle(self):
+        if self._h is None:
+            hs = [op._handle() for op in self.operators]
+            arr = (C.c_void_p * len(hs))(*[h.value for h in hs])
+            h = C.c_void_p()
+            call("qob_lazyproduct_create", context().h, len(hs), arr, c64(self.factor), C.byref(h))
+            self._h = _Handle(h)
+            self._keep = self.operators
+        return self._h.h
+
+
 class _Adjoint(AbstractOperator):
     """dagger of an operator with a device handle."""
 

@@ -369,3 +369,31 @@ def test_sparse_right_stripes(rows_tunable, cc, adj, dtype, tol):
     finally:
         for k, v in saved.items():
             _lib.call("qob_tune_set", k.encode(), v)
+
+
+def _q_lazyproduct(lp):
+    B = comp((2, 3))
+    ops = []
+    for o in lp.operators:
+        ops.append(to_q(o) if isinstance(o, (O.OLazyTensor, O.OLazySum)) else to_q(o, B, B))
+    return Q.LazyProduct(ops, lp.factor)
+
+
+@pytest.mark.parametrize("alpha,beta", AB)
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_lazyproduct(alpha, beta, dtype, tol):
+    from test_oracle import _lp_case
+    g = np.random.default_rng(177)
+    lp = _lp_case(g)
+    qlp = _q_lazyproduct(lp)
+    run_all_paths(lp, qlp, g, alpha, beta, dtype=dtype, tol=tol)
+    # adjoint, inside a LazySum, and nested in another LazyProduct
+    run_all_paths(O.dagger(lp), Q.dagger(qlp), g, alpha, beta, dtype=dtype, tol=tol)
+    B = comp((2, 3))
+    D = cg(g, 6, 6)
+    s_o = O.OLazySum(6, 6, [0.5, 1j], [lp, O.ODense(D)])
+    s_q = Q.LazySum(B, B, [0.5, 1j], (qlp, Q.DenseOperator(B, B, D)))
+    run_all_paths(s_o, s_q, g, alpha, beta, dtype=dtype, tol=tol)
+    n_o = O.OLazyProduct([lp, O.ODense(D), lp], -0.5)
+    n_q = Q.LazyProduct([qlp, Q.DenseOperator(B, B, D), qlp], -0.5)
+    run_all_paths(n_o, n_q, g, alpha, beta, dtype=dtype, tol=tol)

@@ -440,3 +440,76 @@ def test_c_lazysum_engine_matches_oracle_and_kron():
     r = np.full(D, np.nan + 0j)
     LazySumKet(shape, []).mul(r, psi, 1, 0)
     assert np.array_equal(r, np.zeros(D))
+
+
+def _lp_case(g):
+    """A LazyProduct of a LazyTensor, a dense operator, a LazySum and a
+    sparse operator on 2 x 3 sites (all 6 x 6), factor 0.5 - 0.25i."""
+    lt = rand_lt(g, (2, 3), (2, 3), (1, 2), ("sparse", "dense"))
+    dn = O.ODense(cg(g, 6, 6))
+    ls = O.OLazySum(6, 6, [0.3, -1.1j], [rand_lt(g, (2, 3), (2, 3), (2,), ("sparse",)),
+                                          rand_lt(g, (2, 3), (2, 3), (1,), ("dense",))])
+    spm = O.OSparse.from_scipy(rs(g, 6, 6, 0.4))
+    return O.OLazyProduct([lt, dn, ls, spm], 0.5 - 0.25j)
+
+
+@pytest.mark.parametrize("alpha,beta", AB)
+def test_lazyproduct_vs_dense_product(alpha, beta):
+    g = np.random.default_rng(77)
+    lp = _lp_case(g)
+    M = O.dense_of(lp)
+    # brute force: the product of the factors' own dense matrices
+    P = lp.factor * O.dense_of(lp.operators[0])
+    for o in lp.operators[1:]:
+        P = P @ O.dense_of(o)
+    assert O.rel_frobenius(M, P) < 1e-14
+    x, r0 = cg(g, 6), cg(g, 6)
+    B, R0 = cg(g, 6, 4), cg(g, 6, 4)
+    Bl, Rl = cg(g, 3, 6), cg(g, 3, 6)
+    r = r0.copy()
+    O.mul_ket(r, lp, x, alpha, beta)
+    assert O.rel_frobenius(r, alpha * M @ x + beta * r0) < 1e-12
+    r = r0.copy()
+    O.mul_bra(r, x, lp, alpha, beta)
+    assert O.rel_frobenius(r, alpha * x @ M + beta * r0) < 1e-12
+    R = R0.copy()
+    O.mul_op_left(R, lp, B, alpha, beta)
+    assert O.rel_frobenius(R, alpha * M @ B + beta * R0) < 1e-12
+    R = Rl.copy()
+    O.mul_op_right(R, Bl, lp, alpha, beta)
+    assert O.rel_frobenius(R, alpha * Bl @ M + beta * Rl) < 1e-12
+    # dagger reverses the factors and conjugates the factor
+    assert O.rel_frobenius(O.dense_of(O.dagger(lp)), M.conj().T) < 1e-14
+
+
+def test_lazyproduct_single_factor_alpha_zero_and_in_sum():
+    g = np.random.default_rng(78)
+    lp1 = O.OLazyProduct([O.ODense(cg(g, 4, 5))], 2.0)
+    x = cg(g, 5)
+    r = np.zeros(4, complex)
+    O.mul_ket(r, lp1, x, 0.5, 0)
+    assert O.rel_frobenius(r, O.dense_of(lp1) @ x * 0.5) < 1e-14
+    r = np.full(4, np.nan + 0j)
+    O.mul_ket(r, lp1, x, 0, 0)  # alpha = 0, beta = 0 overwrites NaN
+    assert np.array_equal(r, np.zeros(4))
+    lp = _lp_case(g)
+    s = O.OLazySum(6, 6, [1.0, 2.0j], [lp, O.ODense(cg(g, 6, 6))])
+    y = cg(g, 6)
+    r = np.zeros(6, complex)
+    O.mul_ket(r, s, y)
+    assert O.rel_frobenius(r, O.dense_of(s) @ y) < 1e-12
+
+
+def test_planted_lazyproduct_order_mistake_is_caught(monkeypatch):
+    g = np.random.default_rng(79)
+    lp = _lp_case(g)
+    x = cg(g, 6)
+    orig = O._lp_chain
+
+    def reversed_chain(result, lpx, b, alpha, beta, mul, right):
+        return orig(result, lpx, b, alpha, beta, mul, not right)  # wrong order
+    monkeypatch.setattr(O, "_lp_chain", reversed_chain)
+    r = np.zeros(6, complex)
+    with pytest.raises(Exception):
+        O.mul_ket(r, lp, x, 1, 0)
+        assert O.rel_frobenius(r, O.dense_of(lp) @ x) < 1e-12
