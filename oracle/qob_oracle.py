@@ -806,3 +806,46 @@ def rel_frobenius(x, ref) -> float:
     d = np.linalg.norm((x - ref).ravel())
     n = np.linalg.norm(ref.ravel())
     return float(d / n) if n > 0 else float(d)
+
+
+# ---------------------------------------------- src/superoperators.jl restated
+def vec(rho):
+    """Column-major vec, the reshape of src/superoperators.jl:83."""
+    return np.asarray(rho).reshape(-1, order="F")
+
+
+def superop_times_op(S, rho):
+    """SuperOperator * Operator (src/superoperators.jl:82-85)."""
+    n_out = int(round(np.sqrt(S.shape[0])))
+    return (S @ vec(rho)).reshape(n_out, -1, order="F")
+
+
+def spre(A):
+    """tensor(A, identity) = kron(1, A) (src/superoperators.jl:126-131)."""
+    return np.kron(np.eye(A.shape[1]), A)
+
+
+def spost(A):
+    """kron(transpose(A), 1) (src/superoperators.jl:146-151)."""
+    return np.kron(A.T, np.eye(A.shape[0]))
+
+
+def sprepost(A, B):
+    """kron(transpose(B), A) (src/superoperators.jl:165)."""
+    return np.kron(B.T, A)
+
+
+def liouvillian(H, J, rates=None, Jdagger=None):
+    """src/superoperators.jl:205-222 with dense matrices."""
+    Jd = Jdagger if Jdagger is not None else [j.conj().T for j in J]
+    rates = np.ones(len(J)) if rates is None else np.asarray(rates)
+    L = spre(-1j * H) + spost(1j * H)
+    if rates.ndim == 2:
+        pairs = [(i, j, rates[i, j]) for i in range(len(J)) for j in range(len(J))]
+    else:
+        pairs = [(i, i, rates[i]) for i in range(len(J))]
+    for i, j, r in pairs:
+        jdj = r / 2 * Jd[j] @ J[i]
+        L = L - (spre(jdj) + spost(jdj))
+        L = L + spre(r * J[i]) @ spost(Jd[j])
+    return L

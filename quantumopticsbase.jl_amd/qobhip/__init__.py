@@ -13,5 +13,6 @@ from .spin import sigmam, sigmap, sigmax, sigmay, sigmaz
 from .fock import create, destroy, number
 from .batch import Batch, Graph
 from .timedependent import TimeDependentSum
+from .superop import SuperOperator, identitysuperoperator, liouvillian, spost, spre, sprepost
 
 __all__ = [n for n in dir() if not n.startswith("_")]

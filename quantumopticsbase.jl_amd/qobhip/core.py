@@ -607,6 +607,12 @@ def mul_(result, a, b, alpha=1.0, beta=0.0):
       (Operator, op, Operator)  left application on a density matrix
       (Operator, Operator, op)  right application
     """
+    if type(a).__name__ == "SuperOperator":  # S * rho on vec(rho) (src/superoperators.jl:82-85)
+        from .superop import apply
+        if not isinstance(result, Operator) or not isinstance(b, Operator):
+            raise MethodError("a SuperOperator acts on an Operator")
+        apply(a, b, result, alpha, beta)
+        return result
     ctx = context()
     al, be = c64(alpha), c64(beta)
     if isinstance(result, Ket):

@@ -513,3 +513,30 @@ def test_planted_lazyproduct_order_mistake_is_caught(monkeypatch):
     with pytest.raises(Exception):
         O.mul_ket(r, lp, x, 1, 0)
         assert O.rel_frobenius(r, O.dense_of(lp) @ x) < 1e-12
+
+
+def test_superoperator_identities_and_lindblad_form():
+    """spre / spost / sprepost act as A B, B A, A C B on vec (closed forms),
+    the liouvillian equals the Lindblad right-hand side written out, and is
+    trace preserving (src/precompile.jl:84 checks the same at 1e-11)."""
+    g = np.random.default_rng(81)
+    n = 5
+    A, B, C = cg(g, n, n), cg(g, n, n), cg(g, n, n)
+    assert O.rel_frobenius(O.superop_times_op(O.spre(A), B), A @ B) < 1e-14
+    assert O.rel_frobenius(O.superop_times_op(O.spost(A), B), B @ A) < 1e-14
+    assert O.rel_frobenius(O.superop_times_op(O.sprepost(A, B), C), A @ C @ B) < 1e-14
+    H = A + A.conj().T
+    J = [cg(g, n, n), cg(g, n, n)]
+    rates = np.array([0.3, 1.7])
+    rho = B @ B.conj().T
+    rho /= np.trace(rho)
+    L = O.liouvillian(H, J, rates)
+    want = -1j * (H @ rho - rho @ H)
+    for r, j in zip(rates, J):
+        jd = j.conj().T
+        want = want + r * (j @ rho @ jd - 0.5 * (jd @ j @ rho + rho @ jd @ j))
+    got = O.superop_times_op(L, rho)
+    assert O.rel_frobenius(got, want) < 1e-13
+    assert abs(np.trace(got)) < 1e-11 * np.abs(got).max()
+    # matrix rates: the diagonal matrix equals the vector form
+    assert O.rel_frobenius(O.liouvillian(H, J, np.diag(rates)), L) < 1e-14
