@@ -197,6 +197,18 @@ def run_sparse6400(args):
         lz_ms[name] = ev(f, 10)
     ctx.set_mode(Q.QOB_MODE_FUSED)
 
+    # expect(H, rho) = tr(H rho): one pass over the nonzeros (fused) against
+    # the plain form (apply into a temporary, then trace)
+    ex_ms = {}
+    for mode, name in ((Q.QOB_MODE_FUSED, "fused"), (Q.QOB_MODE_PLAIN, "apply_then_trace")):
+        ctx.set_mode(mode)
+        Q.expect(H, rho)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            Q.expect(H, rho)
+        ex_ms[name] = (time.perf_counter() - t0) * 1e3 / 20
+    ctx.set_mode(Q.QOB_MODE_FUSED)
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (C restatement, 1 thread)")
@@ -210,6 +222,7 @@ def run_sparse6400(args):
         "hbm_gbs": {"left": bytes_per_launch / (ms_left * 1e-3) / 1e9,
                     "right": bytes_per_launch / (ms_right * 1e-3) / 1e9},
         "lazysum_form_left_ms": {**lz_ms, "fused_speedup": lz_ms["plain"] / lz_ms["fused"] if "plain" in lz_ms else None},
+        "expect_rho_ms_host_clock": ex_ms,
     }
     roof = {"bound": "hbm", "kernel": "k_sell_left_rr" if dom == "left" else "k_csc_right",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -76,6 +76,20 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
     const int lr = lane & 15, lk = lane >> 4;
     const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
 
+    // factor of the next mode, loaded into registers one mode ahead so the
+    // load latency hides behind the current mode's MFMAs
+    constexpr int APRE = (16 * MAXNTM) * (16 * MAXNTM) / NTH > 0 ? (16 * MAXNTM) * (16 * MAXNTM) / NTH : 1;
+    T apre[APRE];
+    {
+        const T *__restrict__ A0 = (const T *)M.A[0];
+        const int dd = M.d[0] * M.d[0];
+#pragma unroll
+        for (int k = 0; k < APRE; ++k) {
+            const int e = tid + k * NTH;
+            apre[k] = (e < dd) ? A0[e] : czero<T>();
+        }
+    }
+
     T pre[EPT];
     int64_t col = blockIdx.x;
     if (col < ncols) {
@@ -110,11 +124,24 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
             const int n1 = M.n1[s], d = M.d[s];
             const int lda = d + 1;
             __syncthreads();  // previous mode written back, column loaded
+#pragma unroll
+            for (int k = 0; k < APRE; ++k) {
+                const int e = tid + k * NTH;
+                if (e < d * d) {
+                    const int i = e % d, kk = e / d;
+                    Al[i + kk * lda] = apre[k];
+                }
+            }
             {
-                const T *__restrict__ A = (const T *)M.A[s];
-                for (int e = tid; e < d * d; e += NTH) {
-                    const int i = e % d, k = e / d;
-                    Al[i + k * lda] = A[e];
+                // next factor: this column's next mode, else the first mode
+                // of the next column
+                const int sn = (s + 1 < M.nmodes) ? s + 1 : 0;
+                const T *__restrict__ An = (const T *)M.A[sn];
+                const int dd = M.d[sn] * M.d[sn];
+#pragma unroll
+                for (int k = 0; k < APRE; ++k) {
+                    const int e = tid + k * NTH;
+                    apre[k] = (e < dd) ? An[e] : czero<T>();
                 }
             }
             __syncthreads();
@@ -159,19 +186,43 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
                         }
                     }
                 }
-                // the group's inputs are no longer needed by anyone: write back
+                if (s + 1 < M.nmodes) {
+                    // the group's inputs are no longer needed by anyone: write back
 #pragma unroll
-                for (int q = 0; q < GPW; ++q) {
-                    if (g0 + q * NW < ngroups) {
+                    for (int q = 0; q < GPW; ++q) {
+                        if (g0 + q * NW < ngroups) {
 #pragma unroll
-                        for (int mt = 0; mt < MAXNTM; ++mt) {
-                            if (mt < ntm) {
+                            for (int mt = 0; mt < MAXNTM; ++mt) {
+                                if (mt < ntm) {
 #pragma unroll
-                                for (int r = 0; r < 4; ++r) {
-                                    const int a = mt * 16 + mf<R>::row(lane, r);
-                                    const int idx = pad(base[q] + n1 * a);
-                                    re[idx] = accr[q][mt][r];
-                                    im[idx] = acci[q][mt][r];
+                                    for (int r = 0; r < 4; ++r) {
+                                        const int a = mt * 16 + mf<R>::row(lane, r);
+                                        const int idx = pad(base[q] + n1 * a);
+                                        re[idx] = accr[q][mt][r];
+                                        im[idx] = acci[q][mt][r];
+                                    }
+                                }
+                            }
+                        }
+                    }
+                } else {
+                    // last mode: the tiles go from the accumulators straight
+                    // to the output column (out = alpha y + beta out); the
+                    // stores drain while the next column is being computed
+                    T *dst = out + col * (int64_t)D;
+#pragma unroll
+                    for (int q = 0; q < GPW; ++q) {
+                        if (g0 + q * NW < ngroups) {
+#pragma unroll
+                            for (int mt = 0; mt < MAXNTM; ++mt) {
+                                if (mt < ntm) {
+#pragma unroll
+                                    for (int r = 0; r < 4; ++r) {
+                                        const int e = base[q] + n1 * (mt * 16 + mf<R>::row(lane, r));
+                                        T v = cmul(alpha, mk((R)accr[q][mt][r], (R)acci[q][mt][r]));
+                                        if (!beta_zero) v = cfma(beta, dst[e], v);
+                                        st_stream(dst + e, v);
+                                    }
                                 }
                             }
                         }
@@ -179,20 +230,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
                 }
             }
         }
-        __syncthreads();
-
-        T *dst = out + col * (int64_t)D;
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int e = tid + k * NTH;
-            if (e < D) {
-                const T y = mk(re[pad(e)], im[pad(e)]);
-                T r = cmul(alpha, y);
-                if (!beta_zero) r = cfma(beta, dst[e], r);
-                st_stream(dst + e, r);
-            }
-        }
-        __syncthreads();
+        __syncthreads();  // every wave is done reading this column's planes
     }
 }
 

@@ -4,10 +4,12 @@
 Julia binding of libqob_hip.so: device-resident `Ket`/`Bra`/`Operator`
 (data in AMDGPU.jl `ROCArray`s) get `mul!` methods that call the C ABI
 declared in include/qob_hip.h, one `ccall` per `mul!`.  Operators
-(`SparseOperator`, `DenseOperator` factors, `LazyTensor`, `LazySum`) are
-uploaded once and cached by object identity; `LazySum` coefficient changes
-(e.g. `set_time!` of a `TimeDependentSum`) are pushed with
-`qob_lazysum_set_factors` before the next call.
+(`SparseOperator`, `DenseOperator` factors, `LazyTensor`, `LazySum`,
+`LazyProduct`, `TimeDependentSum`) are uploaded once and cached by object
+identity; `LazySum` coefficient changes are pushed with
+`qob_lazysum_set_factors` before the next call, and those of a
+`TimeDependentSum` after `set_time!` with the stream-ordered
+`qob_lazysum_set_factors_d`.
 
 This file is written for QuantumOpticsBase.jl v0.5 with AMDGPU.jl; it is the
 binding a maintainer adds as a package extension (see INTEGRATION.md).  It
@@ -19,7 +21,8 @@ module QobHIP
 using LinearAlgebra, SparseArrays
 using AMDGPU
 using QuantumOpticsBase
-using QuantumOpticsBase: LazyTensor, LazySum, SparseOpPureType, SparseOpAdjType, DataOperator, EyeOpType
+using QuantumOpticsBase: LazyTensor, LazySum, LazyProduct, TimeDependentSum, SparseOpPureType, SparseOpAdjType,
+                         DataOperator, EyeOpType, static_operator
 import LinearAlgebra: mul!
 
 const libqob = Ref{String}(joinpath(@__DIR__, "..", "lib", "libqob_hip.so"))
@@ -164,10 +167,42 @@ function handle(op::LazySum)
     h
 end
 
+# LazyProduct (src/operators_lazyproduct.jl:24-58): factors applied in turn
+# by the library through its own device temporaries
+function handle(op::LazyProduct)
+    cache!(op) do
+        hs = Ptr{Cvoid}[handle(o) for o in op.operators]
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:qob_lazyproduct_create, libqob[]), Cint,
+                    (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, QobC64, Ref{Ptr{Cvoid}}),
+                    context(), length(hs), hs, QobC64(op.factor), h))
+        h[]
+    end
+end
+
+# TimeDependentSum (src/time_dependent_operator.jl:274-277): mul! on the static
+# LazySum.  After set_time! the evaluated coefficients go to the device and
+# the library rewrites its plans there (qob_lazysum_set_factors_d), stream-
+# ordered: no plan rebuild, no device synchronisation.
+const DCOEF = IdDict{Any,ROCVector{ComplexF64}}()
+function handle(op::TimeDependentSum)
+    ls = static_operator(op)
+    h = get(HANDLES, ls, C_NULL)
+    h == C_NULL && return handle(ls)
+    if SUM_FACTORS[ls] != ls.factors
+        d = get!(() -> ROCVector{ComplexF64}(undef, length(ls.factors)), DCOEF, ls)
+        copyto!(d, ComplexF64.(ls.factors))
+        st = ccall((:qob_lazysum_set_factors_d, libqob[]), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Cvoid}),
+                   context(), h, pointer(d), stream_ptr())
+        st == 0 ? (SUM_FACTORS[ls] = ComplexF64.(ls.factors)) : return handle(ls)  # host path otherwise
+    end
+    h
+end
+
 # ------------------------------------------------------------------- mul!
 const DevVec{T} = ROCVector{T}
 const DevMat{T} = ROCMatrix{T}
-const DevOp = Union{LazySum,LazyTensor,SparseOpPureType,SparseOpAdjType}
+const DevOp = Union{LazySum,LazyTensor,LazyProduct,TimeDependentSum,SparseOpPureType,SparseOpAdjType}
 const CT = Union{ComplexF64,ComplexF32}
 
 # mul!(::Ket, op, ::Ket, α, β)  — src/operators_lazysum.jl:189,
@@ -234,6 +269,25 @@ function LinearAlgebra.tr(op::Operator{B,B,<:DevMat{T}}) where {B,T<:CT}
     out = Ref{QobC64}(QobC64(0, 0))
     check(ccall((:qob_trace, libqob[]), Cint, (Ptr{Cvoid}, Cint, Int64, Ptr{T}, Ref{QobC64}, Ptr{Cvoid}),
                 context(), qob_dtype(T), size(op.data, 1), pointer(op.data), out, stream_ptr()))
+    complex(out[].re, out[].im)
+end
+
+# expect(op, rho) = tr(op rho) without forming op rho where possible
+# (src/operators_sparse.jl:36-45, src/operators_dense.jl:223-230)
+function QuantumOpticsBase.expect(op::DevOp, state::Operator{B,B,<:DevMat{T}}) where {B,T<:CT}
+    out = Ref{QobC64}(QobC64(0, 0))
+    check(ccall((:qob_expect_op, libqob[]), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{T}, Int64, Ref{QobC64}, Ptr{Cvoid}),
+                context(), qob_dtype(T), handle(op), pointer(state.data), size(state.data, 1), out, stream_ptr()))
+    complex(out[].re, out[].im)
+end
+
+# variance(op, ket) (src/operators.jl:139-143)
+function QuantumOpticsBase.variance(op::DevOp, state::Ket{B,<:DevVec{T}}) where {B,T<:CT}
+    out = Ref{QobC64}(QobC64(0, 0))
+    check(ccall((:qob_variance_ket, libqob[]), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{T}, Int64, Ref{QobC64}, Ptr{Cvoid}),
+                context(), qob_dtype(T), handle(op), pointer(state.data), length(state.data), out, stream_ptr()))
     complex(out[].re, out[].im)
 end
 
