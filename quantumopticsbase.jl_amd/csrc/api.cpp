@@ -33,7 +33,7 @@ namespace qob {
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
-extern int g_modek_nth, g_fused_gpw, g_csc_rows;
+extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -2015,7 +2015,8 @@ static int *tune_slot(const char *name) {
                {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
                {"qubit_split", &g_qubit_split},   {"modek_nth", &g_modek_nth},
-               {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows}};
+               {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
+               {"sell_unr", &g_sell_unr}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2032,6 +2033,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_tbits") && (value < 10 || value > 13)) return fail(QOB_EARG, "qubit_tbits must be in 10..13");
     if (!strcmp(name, "modek_nth") && value != 64 && value != 128 && value != 256 && value != 512)
         return fail(QOB_EARG, "modek_nth must be 64, 128, 256 or 512");
+    if (!strcmp(name, "sell_unr") && value != 2 && value != 4 && value != 8)
+        return fail(QOB_EARG, "sell_unr must be 2, 4 or 8");
     if (!strcmp(name, "csc_rows") && value != 16 && value != 32 && value != 64)
         return fail(QOB_EARG, "csc_rows must be 16, 32 or 64");
     if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)

@@ -261,6 +261,7 @@ hipError_t launch_csc_trace(qob_dtype dt, const CscDev &X, const void *V, int64_
 int g_sell_variant = 1;  // 0: column groups of CJ, 1: register-resident rows
 int g_sell_cj = 4;
 int g_sell_jchunk = 8;
+int g_sell_unr = 4;  // columns in flight per lane in k_sell_left_rr (2, 4 or 8)
 int g_csc_rb = 64;
 int g_csc_cc = 2;
 int g_csc_remap = 1;
@@ -273,7 +274,13 @@ static void sell_rr(const SellDev &X, const void *B, int64_t cols, void *R, doub
 #define QOB_RR(MW, U)                                                                                          \
     hipLaunchKernelGGL((k_sell_left_rr<T, MW, U>), dim3(G), dim3(256), 0, s, X.nslices, X.m, X.slice_off, X.col, \
                        (const T *)X.val, (const T *)B, X.n, cols, (T *)R, X.m, alpha, beta, bz, nsg, jchunk)
-    if (maxw <= 2) QOB_RR(2, 4);
+    if (g_sell_unr == 8 && maxw <= 4) {
+        if (maxw <= 2) QOB_RR(2, 8);
+        else QOB_RR(4, 8);
+    } else if (g_sell_unr == 2 && maxw <= 4) {
+        if (maxw <= 2) QOB_RR(2, 2);
+        else QOB_RR(4, 2);
+    } else if (maxw <= 2) QOB_RR(2, 4);
     else if (maxw <= 4) QOB_RR(4, 4);
     else if (maxw <= 8) QOB_RR(8, 2);
     else QOB_RR(16, 1);

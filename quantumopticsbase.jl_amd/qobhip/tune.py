@@ -37,7 +37,8 @@ SPACES: Dict[str, Dict[str, List[int]]] = {
     "fused": {"fused_gpw": [1, 2, 4]},
     "qubit": {"qubit_tbits": [10, 11, 12, 13], "qubit_nth": [256, 512, 1024], "qubit_grid": [0, 1024, 2048, 4096],
               "qubit_remap": [0, 1], "qubit_split": [0, 1]},
-    "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16]},
+    "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16, 32],
+                    "sell_unr": [2, 4, 8]},
     "sparse_right": {"csc_rows": [16, 32, 64], "csc_rb": [64, 128, 256], "csc_cc": [1, 2, 4, 8], "csc_remap": [0, 1]},
 }
 
@@ -90,10 +91,12 @@ class EventTimer:
 
 
 def coordinate_search(space: Dict[str, Sequence[int]], start: Dict[str, int], cost: Callable[[Dict[str, int]], float],
-                      passes: int = 2) -> tuple:
+                      passes: int = 2, min_gain: float = 0.0) -> tuple:
     """Minimise cost(config) one coordinate at a time; returns (best, best_cost, trials).
 
-    Every distinct configuration is evaluated once (results are cached)."""
+    A move is taken only when it lowers the cost by more than the fraction
+    ``min_gain`` (timing noise between runs and devices must not flip a
+    setting).  Every distinct configuration is evaluated once (cached)."""
     seen: Dict[tuple, float] = {}
     trials = []
 
@@ -115,7 +118,7 @@ def coordinate_search(space: Dict[str, Sequence[int]], start: Dict[str, int], co
                 cand = dict(best)
                 cand[name] = v
                 c = ev(cand)
-                if c < best_cost:
+                if c < best_cost * (1.0 - min_gain):
                     best, best_cost, changed = cand, c, True
         if not changed:
             break
@@ -123,13 +126,16 @@ def coordinate_search(space: Dict[str, Sequence[int]], start: Dict[str, int], co
 
 
 def tune(family: str, make_run: Callable[[], Callable[[], None]], stream=None, reps: int = 10, warmup: int = 2,
-         passes: int = 2, space: Optional[Dict[str, Sequence[int]]] = None) -> dict:
+         passes: int = 2, space: Optional[Dict[str, Sequence[int]]] = None, repeats: int = 3,
+         min_gain: float = 0.02) -> dict:
     """Tune one family of tunables on a workload.
 
     ``make_run()`` builds the workload under the current tunables (it is
     called again for each trial, since plans and batches depend on them) and
-    returns a callable that enqueues one unit of work on ``stream``.  The
-    tunables are left at the best values found."""
+    returns a callable that enqueues one unit of work on ``stream``.  A
+    trial's cost is the fastest of ``repeats`` timings of ``reps`` calls; a
+    setting changes only for a gain above ``min_gain``.  The tunables are
+    left at the best values found."""
     from .core import context
     space = dict(space or SPACES[family])
     ctx = context()
@@ -141,10 +147,10 @@ def tune(family: str, make_run: Callable[[], Callable[[], None]], stream=None, r
         run = make_run()
         for _ in range(warmup):
             run()
-        return timer(run, reps)
+        return min(timer(run, reps) for _ in range(max(1, repeats)))
 
     try:
-        best, best_ms, trials = coordinate_search(space, start, cost, passes)
+        best, best_ms, trials = coordinate_search(space, start, cost, passes, min_gain)
     except Exception:
         apply(start)
         raise

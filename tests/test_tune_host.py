@@ -58,3 +58,12 @@ def test_invalid_tunable_values_are_rejected():
     for name, bad in [("modek_nth", 100), ("fused_gpw", 3), ("qubit_split", 2), ("qubit_tbits", 9)]:
         with pytest.raises(ArgumentError):
             T.set_(name, bad)
+
+
+def test_min_gain_keeps_start_against_noise():
+    space = {"a": [1, 2, 3]}
+    costs = {1: 1.0, 2: 0.99, 3: 1.2}  # 1% better only
+    best, c, _ = T.coordinate_search(space, {"a": 1}, lambda g: costs[g["a"]], min_gain=0.02)
+    assert best == {"a": 1} and c == 1.0
+    best, _, _ = T.coordinate_search(space, {"a": 1}, lambda g: costs[g["a"]], min_gain=0.0)
+    assert best == {"a": 2}
