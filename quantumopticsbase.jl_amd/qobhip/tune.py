@@ -6,7 +6,7 @@ of libqob_hip.so (qob_tune_set / qob_tune_get):
   mode-k products   modek_nth (fibres per workgroup), fused_gpw (fibre
                     groups in flight per wave in the fused MFMA kernel)
   qubit sums        qubit_tbits (tile = 2^tbits states), qubit_nth,
-                    qubit_grid (persistent workgroups; 0 = 5 per CU),
+                    qubit_grid (persistent workgroups; 0 = 3 per CU for ComplexF64, 4 for ComplexF32),
                     qubit_remap (XCD-contiguous tiles), qubit_split
   sparse products   sell_variant, sell_cj, sell_jchunk, csc_rows, csc_rb,
                     csc_cc, csc_remap, nt_store
@@ -35,7 +35,7 @@ from ._lib import call
 SPACES: Dict[str, Dict[str, List[int]]] = {
     "modek": {"modek_nth": [64, 128, 256, 512]},
     "fused": {"fused_gpw": [1, 2, 4]},
-    "qubit": {"qubit_tbits": [10, 11, 12, 13], "qubit_nth": [256, 512, 1024], "qubit_grid": [0, 1024, 2048, 4096],
+    "qubit": {"qubit_tbits": [10, 11, 12, 13], "qubit_nth": [256, 512, 1024], "qubit_grid": [0, 768, 1024, 2048],
               "qubit_remap": [0, 1], "qubit_split": [0, 1]},
     "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16, 32],
                     "sell_unr": [2, 4, 8]},
