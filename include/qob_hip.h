@@ -219,7 +219,9 @@ typedef struct qob_graph_s *qob_graph;
 #define QOB_BATCH_SLOTS 4
 /* QOB_EUNSUPPORTED if an operator is not a qubit sum of monomial factors. */
 qob_status qob_batch_create(qob_ctx ctx, int n, const qob_op *ops, qob_dtype dt, qob_batch *out);
-/* Bind in[j] / out[j] (device pointers, 2^L_j elements each) to a slot. */
+/* Bind in[j] / out[j] (device pointers, 2^L_j elements each) to a slot.
+ * Re-binding a bound slot waits for the whole device first (applies in flight
+ * on any stream may read the slot's table); a first bind does not wait. */
 qob_status qob_batch_bind(qob_ctx ctx, qob_batch b, int slot, void *const *in, void *const *out);
 /* For every j: out_j = alpha * op_j * in_j + gamma * in_j + beta * out_j
  * (gamma fuses the Ket update of a time step; in_j must not alias out_j). */

@@ -2431,7 +2431,10 @@ qob_status qob_batch_bind(qob_ctx ctx, qob_batch b, int slot, void *const *in, v
         S[j].in = in[j];
         S[j].out = out[j];
     }
-    QOB_HIP(hipStreamSynchronize(ctx->stream));
+    // A slot bound before may still be read by an apply in flight on any stream: wait for the whole
+    // device before the table is overwritten. A first bind has no readers (the copy below is
+    // synchronous with the host).
+    if (b->bound[slot]) QOB_HIP(hipDeviceSynchronize());
     QOB_TRY(b->slot[slot].upload(S.data(), S.size() * sizeof(QubitStateDev)));
     b->bound[slot] = true;
     return QOB_OK;

@@ -34,7 +34,18 @@ class Batch:
         return tot.value, n.value
 
     def bind(self, slot: int, ins: Sequence[DeviceArray], outs: Sequence[DeviceArray]) -> None:
-        """Bind per-state input/output device buffers to a slot."""
+        """Bind per-state input/output device buffers to a slot.
+
+        Re-binding a bound slot waits for the whole device first."""
+        n = len(self.ops)
+        if len(ins) != n or len(outs) != n:
+            raise ValueError(f"bind needs {n} input and {n} output buffers, got {len(ins)} and {len(outs)}")
+        for j, (x, y) in enumerate(zip(ins, outs)):
+            if x.dtype != self.dtype or y.dtype != self.dtype:
+                raise TypeError(f"state {j}: buffers must be {self.dtype}")
+            d = self.ops[j].shape[1]
+            if x.n < d or y.n < d:
+                raise ValueError(f"state {j}: buffers hold fewer than {d} elements")
         pin = (C.c_void_p * len(ins))(*[d.ptr.value for d in ins])
         pout = (C.c_void_p * len(outs))(*[d.ptr.value for d in outs])
         call("qob_batch_bind", context().h, self.h, slot, pin, pout)

@@ -13,6 +13,8 @@ import scipy.sparse as sp
 import qob_oracle as O
 import qobhip as Q
 from qobhip import _lib
+from qobhip._lib import call
+from qobhip.serve import new_stream
 from qobhip import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -163,6 +165,45 @@ def test_batch_apply_and_axpy(dtype, tol):
     bt.axpy(1, 0.3j)
     for b0, i, o in zip(before, ins, outs):
         assert O.rel_frobenius(i.to_numpy(), b0 + 0.3j * o.to_numpy()) < tol
+
+
+def test_batch_rebind_while_busy_on_another_stream():
+    """Re-binding a slot while applies that read it are queued on another
+    stream: the queued applies use the old buffers, later ones the new."""
+    Ls = [10, 14, 16]
+    specs, ops, psis = _tfim_states(Ls, 31)
+    bt = Q.Batch(ops)
+    s = new_stream()
+    ins = [Q.DeviceArray.from_numpy(p) for p in psis]
+    outs_a = [Q.DeviceArray(2 ** L) for L in Ls]
+    outs_b = [Q.DeviceArray(2 ** L) for L in Ls]
+    bt.bind(0, ins, outs_a)
+    for _ in range(20):
+        bt.apply(0, 1.0, 0.0, 0.0, s)
+    bt.bind(0, ins, outs_b)
+    bt.apply(0, 2.0, 0.0, 0.0, s)
+    call("qob_stream_synchronize", Q.context().h, s)
+    call("qob_stream_destroy", Q.context().h, s)
+    for spec, p, oa, ob, L in zip(specs, psis, outs_a, outs_b, Ls):
+        want = np.zeros(2 ** L, complex)
+        O.mul_ket(want, _oracle_of(spec), p, 1.0, 0.0)
+        assert O.rel_frobenius(oa.to_numpy(), want) < 1e-12
+        assert O.rel_frobenius(ob.to_numpy(), 2 * want) < 1e-12
+
+
+def test_batch_bind_checks_buffers():
+    Ls = [4, 6]
+    specs, ops, psis = _tfim_states(Ls, 3)
+    bt = Q.Batch(ops)
+    ins = [Q.DeviceArray.from_numpy(p) for p in psis]
+    outs = [Q.DeviceArray(2 ** L) for L in Ls]
+    with pytest.raises(ValueError):
+        bt.bind(0, ins[:1], outs[:1])
+    with pytest.raises(ValueError):
+        bt.bind(0, ins, [outs[0], Q.DeviceArray(8)])
+    with pytest.raises(TypeError):
+        bt.bind(0, ins, [outs[0], Q.DeviceArray(2 ** 6, np.complex64)])
+    bt.bind(0, ins, outs)
 
 
 def test_batch_rejects_non_qubit_operator():
