@@ -242,11 +242,25 @@ qob_status qob_graph_destroy(qob_graph g);
 /* ------------------------------------------------------------- tuning
  * Launch-shape knobs read by the kernels (no counterpart in the reference:
  * the reference has no device launch shapes).  Names: "sell_variant",
- * "sell_cj", "sell_jchunk", "csc_rb", "csc_cc", "csc_remap".  The project's
+ * "sell_cj", "sell_jchunk", "sell_unr", "csc_rb", "csc_cc", "csc_remap",
+ * "csc_rows", "csc_order", "qubit_tbits", "qubit_nth", "qubit_grid",
+ * "qubit_remap", "qubit_split", "qubit_enable", "modek_nth", "fused_gpw",
+ * "fused_lt", "nt_store", "copy_variant".  The project's
  * autotuner (qobhip/tune.py) sweeps them and persists a table; unknown names
  * return QOB_EARG.  Settings are process-wide. */
 qob_status qob_tune_set(const char *name, int64_t value);
 qob_status qob_tune_get(const char *name, int64_t *value);
+
+/* Column traversal order the right product R = B X uses for a sparse X
+ * (no counterpart in the reference; host-only, needs no device).  X is given
+ * as CSC (colptr[n + 1], rowind[nnz], rows 0..xrows-1).  On return *used is 1
+ * and order[0..n) is the reverse Cuthill-McKee order of the pattern of X^T X
+ * when the library sweeps columns in that order, else *used is 0 and order
+ * is the identity (the natural order is kept when it already places columns
+ * that share a row of X within 64 positions, or the reordering does not at
+ * least halve that distance). */
+qob_status qob_csc_column_order(int64_t n, int64_t xrows, const int32_t *colptr, const int32_t *rowind,
+                                int32_t *order, int *used);
 
 /* -------------------------------------------- elementwise and reductions
  * Reductions accumulate in FP64 for both dtypes.  The host-result variants

@@ -33,7 +33,7 @@ namespace qob {
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
-extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr;
+extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -171,7 +171,7 @@ struct SellTab {
     SellDev dev;
 };
 struct CscTab {
-    DevBuf ptr, row, val;
+    DevBuf ptr, row, val, ord;
     CscDev dev;
 };
 
@@ -230,6 +230,120 @@ static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
     return QOB_OK;
 }
 
+// Traversal order of the output columns for the right product R = B X.
+// Output columns c and c' that share a row k of X both read column k of B;
+// the pairs are the off-diagonal pattern of X^T X.  The kernel sweeps a row
+// stripe of B per XCD across the columns in traversal order, so the second
+// read of a column of B hits that XCD's L2 only if the two columns are close
+// in the order.  Reverse Cuthill-McKee on the pattern of X^T X (one
+// breadth-first pass per connected component, from a pseudo-peripheral
+// column, neighbours by increasing degree) minimises that distance.  Returns
+// an empty vector (natural order) when the natural order already keeps the
+// distance within 64 columns, when the reordering does not at least halve it,
+// or when X^T X would have more than 2^26 entries to visit.
+static std::vector<int32_t> csc_column_order(const RowLists &Xcols, int64_t xrows) {
+    const int64_t n = Xcols.nr;
+    if (n < 2 || xrows < 1) return {};
+    std::vector<int32_t> rp((size_t)xrows + 1, 0);  // rows of X as lists of columns
+    for (int32_t r : Xcols.idx) rp[(size_t)r + 1]++;
+    int64_t visits = 0;
+    for (int64_t r = 0; r < xrows; ++r) {
+        visits += (int64_t)rp[r + 1] * rp[r + 1];
+        rp[r + 1] += rp[r];
+    }
+    if (visits > (1ll << 26)) return {};
+    std::vector<int32_t> rc(Xcols.idx.size()), cur(rp.begin(), rp.end() - 1);
+    for (int64_t c = 0; c < n; ++c)
+        for (int32_t p = Xcols.ptr[c]; p < Xcols.ptr[c + 1]; ++p) rc[cur[Xcols.idx[p]]++] = (int32_t)c;
+    std::vector<int64_t> ap((size_t)n + 1, 0);
+    std::vector<int32_t> adj, mark((size_t)n, -1);
+    int64_t bw_nat = 0;
+    for (int64_t c = 0; c < n; ++c) {
+        for (int32_t p = Xcols.ptr[c]; p < Xcols.ptr[c + 1]; ++p) {
+            const int32_t r = Xcols.idx[p];
+            for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
+                const int32_t c2 = rc[q];
+                if (c2 != c && mark[c2] != (int32_t)c) {
+                    mark[c2] = (int32_t)c;
+                    adj.push_back(c2);
+                    bw_nat = std::max<int64_t>(bw_nat, std::abs((int64_t)c2 - c));
+                }
+            }
+        }
+        ap[c + 1] = (int64_t)adj.size();
+    }
+    if (bw_nat <= 64) return {};
+    auto deg = [&](int64_t c) { return ap[c + 1] - ap[c]; };
+    // breadth-first levels from s inside its component: returns the last level
+    std::vector<int32_t> lvl((size_t)n, -1), q;
+    auto levels = [&](int32_t s, std::vector<int32_t> &comp) -> int {
+        comp.clear();
+        comp.push_back(s);
+        lvl[s] = 0;
+        int depth = 0;
+        for (size_t h = 0; h < comp.size(); ++h) {
+            const int32_t u = comp[h];
+            depth = std::max(depth, lvl[u]);
+            for (int64_t e = ap[u]; e < ap[u + 1]; ++e)
+                if (lvl[adj[e]] < 0) {
+                    lvl[adj[e]] = lvl[u] + 1;
+                    comp.push_back(adj[e]);
+                }
+        }
+        return depth;
+    };
+    std::vector<int32_t> order, comp, nb;
+    order.reserve((size_t)n);
+    std::vector<char> done((size_t)n, 0);
+    for (int64_t c0 = 0; c0 < n; ++c0) {
+        if (done[c0]) continue;
+        // pseudo-peripheral start: move to a minimum-degree column of the last
+        // level while that increases the depth (at most 4 moves)
+        int32_t s = (int32_t)c0;
+        int depth = levels(s, comp);
+        for (int it = 0; it < 4; ++it) {
+            int32_t best = -1;
+            for (int32_t u : comp)
+                if (lvl[u] == depth && (best < 0 || deg(u) < deg(best))) best = u;
+            for (int32_t u : comp) lvl[u] = -1;
+            const int d2 = levels(best, comp);
+            if (d2 <= depth) {
+                for (int32_t u : comp) lvl[u] = -1;
+                levels(s, comp);
+                break;
+            }
+            s = best;
+            depth = d2;
+        }
+        for (int32_t u : comp) lvl[u] = -1;
+        // Cuthill-McKee pass over the component
+        const size_t first = order.size();
+        order.push_back(s);
+        done[s] = 1;
+        for (size_t h = first; h < order.size(); ++h) {
+            const int32_t u = order[h];
+            nb.clear();
+            for (int64_t e = ap[u]; e < ap[u + 1]; ++e)
+                if (!done[adj[e]]) {
+                    done[adj[e]] = 1;
+                    nb.push_back(adj[e]);
+                }
+            std::sort(nb.begin(), nb.end(), [&](int32_t a, int32_t b) {
+                return deg(a) != deg(b) ? deg(a) < deg(b) : a < b;
+            });
+            order.insert(order.end(), nb.begin(), nb.end());
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    std::vector<int64_t> pos((size_t)n);
+    for (int64_t k = 0; k < n; ++k) pos[order[k]] = k;
+    int64_t bw = 0;
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t e = ap[c]; e < ap[c + 1]; ++e) bw = std::max<int64_t>(bw, std::abs(pos[c] - pos[adj[e]]));
+    if (2 * bw > bw_nat) return {};
+    return order;
+}
+
 static qob_status build_csc(const RowLists &Xcols, int64_t xrows, qob_dtype dt, CscTab &C) {
     // Xcols: columns of X as lists (i.e. rows of X^T)
     QOB_TRY(C.ptr.upload(Xcols.ptr.data(), Xcols.ptr.size() * 4));
@@ -240,6 +354,11 @@ static qob_status build_csc(const RowLists &Xcols, int64_t xrows, qob_dtype dt, 
     C.dev.colptr = (const int32_t *)C.ptr.p;
     C.dev.row = (const int32_t *)C.row.p;
     C.dev.val = C.val.p;
+    const std::vector<int32_t> ord = csc_column_order(Xcols, xrows);
+    if (!ord.empty()) {
+        QOB_TRY(C.ord.upload(ord.data(), ord.size() * 4));
+        C.dev.corder = (const int32_t *)C.ord.p;
+    }
     return QOB_OK;
 }
 
@@ -2016,7 +2135,7 @@ static int *tune_slot(const char *name) {
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
                {"qubit_split", &g_qubit_split},   {"modek_nth", &g_modek_nth},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
-               {"sell_unr", &g_sell_unr}};
+               {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2040,11 +2159,33 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)
         return fail(QOB_EARG, "fused_gpw must be 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
+    if (!strcmp(name, "csc_order") && value > 1) return fail(QOB_EARG, "csc_order must be 0 or 1");
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
         return fail(QOB_EARG, "qubit_nth must be 256, 512 or 1024");
     if ((!strcmp(name, "csc_cc") || !strcmp(name, "sell_jchunk")) && value < 1)
         return fail(QOB_EARG, "%s must be >= 1", name);
     *p = (int)value;
+    return QOB_OK;
+}
+
+qob_status qob_csc_column_order(int64_t n, int64_t xrows, const int32_t *colptr, const int32_t *rowind,
+                                int32_t *order, int *used) {
+    if (n < 0 || xrows < 0 || (n > 0 && (!colptr || !order)) || !used) return fail(QOB_EARG, "null argument");
+    if (n >= (1ll << 31) || xrows >= (1ll << 31)) return fail(QOB_EARG, "dimension out of range");
+    RowLists X;
+    X.nr = n;
+    X.nc = xrows;
+    X.ptr.assign(colptr, colptr + n + 1);
+    if (X.ptr[0] != 0) return fail(QOB_EARG, "colptr[0] must be 0");
+    for (int64_t c = 0; c < n; ++c)
+        if (X.ptr[c + 1] < X.ptr[c]) return fail(QOB_EARG, "colptr not monotone at %lld", (long long)c);
+    if (X.ptr[n] > 0 && !rowind) return fail(QOB_EARG, "null rowind");
+    X.idx.assign(rowind, rowind + X.ptr[n]);
+    for (int32_t r : X.idx)
+        if (r < 0 || r >= xrows) return fail(QOB_EARG, "row index %d out of range", r);
+    const std::vector<int32_t> ord = csc_column_order(X, xrows);
+    *used = ord.empty() ? 0 : 1;
+    for (int64_t c = 0; c < n; ++c) order[c] = ord.empty() ? (int32_t)c : ord[c];
     return QOB_OK;
 }
 

@@ -139,12 +139,17 @@ __global__ void __launch_bounds__(256) k_sell_left_rr(int nslices, int64_t m, co
 }
 
 // ---- right: CSC column gather, lanes along rows of B/R.
-// Logical block = (row block rb, column group cg), column group fastest.
+// Logical block = (row block rb, column group cg), column group fastest, so
+// one XCD sweeps a stripe of rows across the columns.  Column slot k of the
+// sweep is output column corder[k] when a traversal order is given: an order
+// that puts columns sharing a column of B next to each other makes the
+// second read of that column an L2 hit.
 template <typename T, int RB>
 __global__ void __launch_bounds__(RB) k_csc_right(int64_t rows, int64_t n, const int32_t *__restrict__ colptr,
                                                   const int32_t *__restrict__ rowind, const T *__restrict__ val,
                                                   const T *__restrict__ B, T *__restrict__ R, double2 alpha_,
-                                                  double2 beta_, int beta_zero, uint32_t ncg, int cc, int remap) {
+                                                  double2 beta_, int beta_zero, uint32_t ncg, int cc, int remap,
+                                                  const int32_t *__restrict__ corder) {
     const uint32_t G = gridDim.x;
     const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
     const uint32_t cg = lb % ncg, rb = lb / ncg;
@@ -153,7 +158,8 @@ __global__ void __launch_bounds__(RB) k_csc_right(int64_t rows, int64_t n, const
     const bool ok = i < rows;
     const int64_t c0 = (int64_t)cg * cc;
     const int64_t c1 = (c0 + cc < n) ? c0 + cc : n;
-    for (int64_t c = c0; c < c1; ++c) {
+    for (int64_t k = c0; k < c1; ++k) {
+        const int64_t c = corder ? (int64_t)corder[k] : k;
         const int32_t p0 = colptr[c], p1 = colptr[c + 1];
         T acc = czero<T>();
         if (ok) {
@@ -266,6 +272,7 @@ int g_csc_rb = 64;
 int g_csc_cc = 2;
 int g_csc_remap = 1;
 int g_csc_rows = 64;  // 64: one column per wave step (k_csc_right); 16 or 32: short stripes
+int g_csc_order = 1;  // 1: k_csc_right sweeps columns in the table's traversal order when it has one
 int g_nt_store = 1;
 
 template <typename T>
@@ -366,7 +373,8 @@ hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_
     if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
 #define QOB_CSC(TT, RBV)                                                                                  \
     hipLaunchKernelGGL((k_csc_right<TT, RBV>), dim3((uint32_t)G), dim3(RBV), 0, s, rows, X.n, X.colptr, X.row, \
-                       (const TT *)X.val, (const TT *)B, (TT *)R, alpha, beta, bz, ncg, cc, g_csc_remap)
+                       (const TT *)X.val, (const TT *)B, (TT *)R, alpha, beta, bz, ncg, cc, g_csc_remap, \
+                       g_csc_order ? X.corder : nullptr)
     if (dt == QOB_C64) {
         if (rb == 64) QOB_CSC(double2, 64);
         else if (rb == 128) QOB_CSC(double2, 128);

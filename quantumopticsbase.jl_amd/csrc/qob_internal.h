@@ -171,6 +171,7 @@ struct CscDev {
     int64_t m = 0, n = 0;
     const int32_t *colptr = nullptr, *row = nullptr;
     const void *val = nullptr;
+    const int32_t *corder = nullptr;  // traversal order of the columns (nullptr: natural)
 };
 // R (m x cols) = alpha * X * B + beta * R, B (n x cols)
 hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols,

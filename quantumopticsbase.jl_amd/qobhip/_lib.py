@@ -108,6 +108,7 @@ _PROTOS = {
     "qob_copy": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp]),
     "qob_tune_set": (_i32, [C.c_char_p, _i64]),
     "qob_tune_get": (_i32, [C.c_char_p, _P(_i64)]),
+    "qob_csc_column_order": (_i32, [_i64, _i64, _vp, _vp, _vp, _P(C.c_int)]),
     "qob_convert": (_i32, [_vp, _i32, _vp, _i32, _vp, _i64, _vp]),
     "qob_dot": (_i32, [_vp, _i32, _i64, _vp, _vp, _P(qob_c64), _vp]),
     "qob_norm": (_i32, [_vp, _i32, _i64, _vp, _P(_dbl), _vp]),

@@ -39,7 +39,8 @@ SPACES: Dict[str, Dict[str, List[int]]] = {
               "qubit_remap": [0, 1], "qubit_split": [0, 1]},
     "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16, 32],
                     "sell_unr": [2, 4, 8]},
-    "sparse_right": {"csc_rows": [16, 32, 64], "csc_rb": [64, 128, 256], "csc_cc": [1, 2, 4, 8], "csc_remap": [0, 1]},
+    "sparse_right": {"csc_rows": [16, 32, 64], "csc_rb": [64, 128, 256], "csc_cc": [1, 2, 4, 8], "csc_remap": [0, 1],
+                     "csc_order": [1, 0]},
 }
 
 

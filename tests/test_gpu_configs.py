@@ -88,6 +88,29 @@ def test_fock_spin_sparse_left_right(fock_spin):
     assert O.rel_frobenius(left.conj().T, right) < 1e-12
 
 
+def test_fock_spin_right_column_order_bit_exact(fock_spin):
+    """The right product sweeps columns in reverse Cuthill-McKee order; each
+    output column is still summed in the same order, so the result equals
+    the natural sweep bit for bit."""
+    import ctypes as C
+    from qobhip import _lib
+    spec, rho = fock_spin
+    n = spec.H.shape[0]
+    H = Q.SparseOperator(spec.basis, spec.basis, spec.H)
+    outs = []
+    v = C.c_int64()
+    _lib.call("qob_tune_get", b"csc_order", C.byref(v))
+    try:
+        for flag in (1, 0):
+            _lib.call("qob_tune_set", b"csc_order", flag)
+            out = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
+            Q.mul_(out, rho, H, 0.5 - 0.25j, 0)
+            outs.append(out.to_numpy())
+    finally:
+        _lib.call("qob_tune_set", b"csc_order", v.value)
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_fock_spin_lazy_form_equals_sparse(fock_spin):
     spec, rho = fock_spin
     n = spec.H.shape[0]

@@ -371,6 +371,42 @@ def test_sparse_right_stripes(rows_tunable, cc, adj, dtype, tol):
             _lib.call("qob_tune_set", k.encode(), v)
 
 
+@pytest.mark.parametrize("adj", [False, True])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_sparse_right_reordered_columns(adj, dtype, tol):
+    """rho * X for a banded X with rows and columns relabelled at random, so
+    the right kernel sweeps columns in its reverse Cuthill-McKee order (the
+    order is checked to be in use)."""
+    import ctypes as C
+    from qobhip import _lib
+    g = np.random.default_rng(57)
+    n, rows = 900, 70
+    vals = [g.standard_normal(n - k) + 1j * g.standard_normal(n - k) for k in (2, 0, 1)]
+    Xb = sp.diags(vals, [-2, 0, 1], format="csc")
+    p = g.permutation(n)
+    X = sp.csc_matrix(Xb[p][:, p])
+    X.sort_indices()
+    oop = O.OSparse.from_scipy(X)
+    if adj:
+        oop = O.dagger(oop)
+    # the order is built from the columns the kernel sweeps: those of op
+    cols = sp.csc_matrix(X.conj().T) if adj else X
+    cols.sort_indices()
+    order, used = np.empty(n, np.int32), C.c_int()
+    _lib.call("qob_csc_column_order", n, n, np.ascontiguousarray(cols.indptr, np.int32).ctypes.data,
+              np.ascontiguousarray(cols.indices, np.int32).ctypes.data, order.ctypes.data, C.byref(used))
+    assert used.value == 1
+    qop = to_q(oop)
+    B = g.standard_normal((rows, n)) + 1j * g.standard_normal((rows, n))
+    for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
+        R0 = np.full((rows, n), np.nan, complex) if beta == 0 else g.standard_normal((rows, n)) + 0j
+        want = np.zeros((rows, n), complex) if beta == 0 else R0.copy()
+        O.mul_op_right(want, B, oop, alpha, beta)
+        R = Q.Operator(gb(rows), gb(n), R0, dtype)
+        Q.mul_(R, Q.Operator(gb(rows), gb(n), B, dtype), qop, alpha, beta)
+        assert rel(R.to_numpy(), want) < tol
+
+
 def _q_lazyproduct(lp):
     B = comp((2, 3))
     ops = []
