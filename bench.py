@@ -179,6 +179,12 @@ def run_sparse6400(args):
     reps = max(20, min(200, args.steps))
     ms_left = ev(left, reps)
     ms_right = ev(right, reps)
+    # the same bytes moved by the library's streaming copy kernel (read rho,
+    # write out): the practical ceiling for a kernel that reads and writes
+    # N^2 elements once
+    from qobhip import _lib
+    copy = lambda: _lib.call("qob_copy", ctx.h, 0, n * n, rho.data.ptr, outL.data.ptr, None)
+    ms_copy = ev(copy, reps)
     bytes_per_launch = 32 * n * n + nnz * 20 + (n + 1) * 4
     dom = "left" if ms_left >= ms_right else "right"
     ms_dom = max(ms_left, ms_right)
@@ -218,9 +224,10 @@ def run_sparse6400(args):
         "workload": "SparseOperator mul! left + right on a 6400x6400 ComplexF64 density matrix "
                     "(Fock(39) x Fock(39) x Spin(3/2), nnz 15522); 1 step = 2 applies",
         "N": n, "nnz": nnz, "element": "ComplexF64",
-        "kernel_ms": {"left": ms_left, "right": ms_right},
+        "kernel_ms": {"left": ms_left, "right": ms_right, "copy_same_bytes": ms_copy},
         "hbm_gbs": {"left": bytes_per_launch / (ms_left * 1e-3) / 1e9,
-                    "right": bytes_per_launch / (ms_right * 1e-3) / 1e9},
+                    "right": bytes_per_launch / (ms_right * 1e-3) / 1e9,
+                    "copy": 32 * n * n / (ms_copy * 1e-3) / 1e9},
         "lazysum_form_left_ms": {**lz_ms, "fused_speedup": lz_ms["plain"] / lz_ms["fused"] if "plain" in lz_ms else None},
         "expect_rho_ms_host_clock": ex_ms,
     }

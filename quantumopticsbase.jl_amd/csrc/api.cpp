@@ -34,6 +34,7 @@ extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_r
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
 extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order;
+extern int g_sell_align;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -194,23 +195,49 @@ struct SparseData {
     }
 };
 
+// Slots of a 64-row slice.  Packed: a row's entries fill slots 0, 1, ... in
+// column order, so one slot mixes diagonals (offsets column - row) across
+// the rows of the slice and one load instruction of the kernel touches
+// several separate runs of B.  Diagonal-aligned (when the slice holds at
+// most 16 distinct offsets and no more than twice its widest row): slot k
+// holds offset d_k in every row that has it, so the lanes of a slot read one
+// contiguous run of a column of B.  Rows without an entry on d_k keep the
+// empty marker (-1) and read nothing.
+int g_sell_align = 1;
+
 static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
     const int64_t m = X.nr;
     const int nslices = (int)((m + 63) / 64);
     std::vector<int32_t> off(nslices + 1, 0);
+    std::vector<std::vector<int64_t>> diag(nslices);  // aligned slices: their offsets, ascending
     for (int k = 0; k < nslices; ++k) {
         int w = 0;
-        for (int64_t r = (int64_t)k * 64; r < std::min<int64_t>(m, (int64_t)k * 64 + 64); ++r)
-            w = std::max(w, X.ptr[r + 1] - X.ptr[r]);
+        const int64_t r0 = (int64_t)k * 64, r1 = std::min<int64_t>(m, r0 + 64);
+        for (int64_t r = r0; r < r1; ++r) w = std::max(w, X.ptr[r + 1] - X.ptr[r]);
+        if (g_sell_align && w > 0) {
+            std::vector<int64_t> d;
+            for (int64_t r = r0; r < r1; ++r)
+                for (int32_t q = X.ptr[r]; q < X.ptr[r + 1]; ++q) d.push_back((int64_t)X.idx[q] - r);
+            std::sort(d.begin(), d.end());
+            d.erase(std::unique(d.begin(), d.end()), d.end());
+            if ((int)d.size() <= 16 && (int)d.size() <= 2 * w) {
+                w = (int)d.size();
+                diag[k] = std::move(d);
+            }
+        }
         off[k + 1] = off[k] + w * 64;
     }
     std::vector<int32_t> col(off[nslices], -1);
     std::vector<cd> val(off[nslices], cd(0, 0));
     for (int k = 0; k < nslices; ++k) {
+        const std::vector<int64_t> &d = diag[k];
         for (int64_t r = (int64_t)k * 64; r < std::min<int64_t>(m, (int64_t)k * 64 + 64); ++r) {
             int p = 0;
             for (int32_t q = X.ptr[r]; q < X.ptr[r + 1]; ++q, ++p) {
-                const int64_t e = off[k] + (int64_t)p * 64 + (r - (int64_t)k * 64);
+                const int slot = d.empty() ? p
+                                           : (int)(std::lower_bound(d.begin(), d.end(), (int64_t)X.idx[q] - r) -
+                                                   d.begin());
+                const int64_t e = off[k] + (int64_t)slot * 64 + (r - (int64_t)k * 64);
                 col[e] = X.idx[q];
                 val[e] = X.val[q];
             }
@@ -2135,7 +2162,8 @@ static int *tune_slot(const char *name) {
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
                {"qubit_split", &g_qubit_split},   {"modek_nth", &g_modek_nth},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
-               {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order}};
+               {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
+               {"sell_align", &g_sell_align}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2159,7 +2187,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)
         return fail(QOB_EARG, "fused_gpw must be 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
-    if (!strcmp(name, "csc_order") && value > 1) return fail(QOB_EARG, "csc_order must be 0 or 1");
+    if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align")) && value > 1)
+        return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
         return fail(QOB_EARG, "qubit_nth must be 256, 512 or 1024");
     if ((!strcmp(name, "csc_cc") || !strcmp(name, "sell_jchunk")) && value < 1)

@@ -111,6 +111,28 @@ def test_fock_spin_right_column_order_bit_exact(fock_spin):
     assert np.array_equal(outs[0], outs[1])
 
 
+def test_fock_spin_left_aligned_table_bit_exact(fock_spin):
+    """Diagonal-aligned and packed slot tables of the left kernel sum each
+    row in the same (column) order: equal results bit for bit."""
+    import ctypes as C
+    from qobhip import _lib
+    spec, rho = fock_spin
+    n = spec.H.shape[0]
+    outs = []
+    v = C.c_int64()
+    _lib.call("qob_tune_get", b"sell_align", C.byref(v))
+    try:
+        for flag in (1, 0):
+            _lib.call("qob_tune_set", b"sell_align", flag)
+            H = Q.SparseOperator(spec.basis, spec.basis, spec.H)
+            out = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
+            Q.mul_(out, H, rho, 0.5 - 0.25j, 0)
+            outs.append(out.to_numpy())
+    finally:
+        _lib.call("qob_tune_set", b"sell_align", v.value)
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_fock_spin_lazy_form_equals_sparse(fock_spin):
     spec, rho = fock_spin
     n = spec.H.shape[0]
