@@ -17,7 +17,8 @@
  *                         and the recursion _gemm_recursive_lazy_dense (:652-685)
  * Complex numbers are interleaved doubles, matrices column-major, CSC index
  * arrays 1-based int64 exactly as Julia holds them.  Single-threaded, like the
- * reference's hand-written loops.
+ * reference's hand-written loops, except the _omp variants of the two sparse
+ * products (an all-cores upper bound).
  */
 #include <complex.h>
 #include <stdint.h>
@@ -58,6 +59,46 @@ void qobref_gemm_dense_sp(double alpha_re, double alpha_im, int64_t rows, int64_
     const int one = (alpha == 1);
     for (int64_t c = 0; c < n; ++c) {
         cplx *rc = result + c * rows;
+        for (int64_t i = colptr[c] - 1; i < colptr[c + 1] - 1; ++i) {
+            const cplx mi = one ? nzval[i] : nzval[i] * alpha;
+            const cplx *bcol = B + (rowval[i] - 1) * rows;
+            for (int64_t j = 0; j < rows; ++j) rc[j] += mi * bcol[j];
+        }
+    }
+}
+
+/* The same two loops with their outer loop over output columns (independent
+ * in both: column j of the result depends on column j of B alone on the left,
+ * on the columns of B that column c of M names on the right) spread over
+ * nthreads OpenMP threads, beta applied per column.  An all-cores upper bound
+ * for the CPU path; the reference's loops themselves are single-threaded. */
+void qobref_gemm_sp_dense_omp(double alpha_re, double alpha_im, int64_t m, int64_t n, const int64_t *colptr,
+                              const int64_t *rowval, const cplx *nzval, const cplx *B, int64_t ncols,
+                              double beta_re, double beta_im, cplx *result, int nthreads) {
+    const cplx alpha = alpha_re + alpha_im * I, beta = beta_re + beta_im * I;
+    const int one = (alpha == 1);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t j = 0; j < ncols; ++j) {
+        const cplx *bj = B + j * n;
+        cplx *rj = result + j * m;
+        beta_prep(beta, rj, m);
+        for (int64_t c = 0; c < n; ++c) {
+            const cplx m2 = one ? bj[c] : alpha * bj[c];
+            for (int64_t i = colptr[c] - 1; i < colptr[c + 1] - 1; ++i) rj[rowval[i] - 1] += nzval[i] * m2;
+        }
+    }
+}
+
+void qobref_gemm_dense_sp_omp(double alpha_re, double alpha_im, int64_t rows, int64_t m, int64_t n,
+                              const int64_t *colptr, const int64_t *rowval, const cplx *nzval, const cplx *B,
+                              double beta_re, double beta_im, cplx *result, int nthreads) {
+    const cplx alpha = alpha_re + alpha_im * I, beta = beta_re + beta_im * I;
+    (void)m;
+    const int one = (alpha == 1);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t c = 0; c < n; ++c) {
+        cplx *rc = result + c * rows;
+        beta_prep(beta, rc, rows);
         for (int64_t i = colptr[c] - 1; i < colptr[c + 1] - 1; ++i) {
             const cplx mi = one ? nzval[i] : nzval[i] * alpha;
             const cplx *bcol = B + (rowval[i] - 1) * rows;

@@ -20,6 +20,8 @@ def lib():
         L = C.CDLL(str(Path(__file__).resolve().parent / "lib" / "libqob_cpu_ref.so"))
         L.qobref_gemm_sp_dense.argtypes = [_dbl, _dbl, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _dbl, _dbl, _vp]
         L.qobref_gemm_dense_sp.argtypes = [_dbl, _dbl, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp]
+        L.qobref_gemm_sp_dense_omp.argtypes = L.qobref_gemm_sp_dense.argtypes + [_int]
+        L.qobref_gemm_dense_sp_omp.argtypes = L.qobref_gemm_dense_sp.argtypes + [_int]
         L.qobref_lazysum_ket.argtypes = [_dbl, _dbl, _int, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _dbl, _dbl, _vp]
         _LIB = L

@@ -442,6 +442,58 @@ def test_c_lazysum_engine_matches_oracle_and_kron():
     assert np.array_equal(r, np.zeros(D))
 
 
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.3 - 0.7j, 0.2 + 0.1j), (0.5, 1.0)])
+def test_c_sparse_products_match_scipy(alpha, beta):
+    """oracle/cpu_ref.c's sparse products (the CPU baseline of the sparse
+    workload) against scipy's CSC products; beta = 0 overwrites NaN; the
+    OpenMP variants sum every output element in the same order, so they
+    equal the one-thread loops bit for bit."""
+    import ctypes as C
+    import scipy.sparse as sp
+    from cref import lib
+    L = lib()
+    g = np.random.default_rng(19)
+    m, n, k = 70, 90, 13
+    M = sp.random(m, n, density=0.15, random_state=3, format="csc") * (1 - 0.5j)
+    M = M + sp.random(m, n, density=0.05, random_state=4, format="csc") * 1j
+    M = sp.csc_matrix(M)
+    M.sort_indices()
+    cp = (M.indptr.astype(np.int64) + 1)
+    rv = (M.indices.astype(np.int64) + 1)
+    nz = np.ascontiguousarray(M.data.astype(np.complex128))
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    a, b = complex(alpha), complex(beta)
+    # left: R (m x k) = a M B + b R, B (n x k)
+    B = np.asfortranarray(g.standard_normal((n, k)) + 1j * g.standard_normal((n, k)))
+    R0 = np.full((m, k), np.nan, complex) if b == 0 else g.standard_normal((m, k)) + 1j * g.standard_normal((m, k))
+    want = a * (M @ B) + (0 if b == 0 else b * R0)
+    outs = []
+    for omp in (0, 1):
+        R = np.asfortranarray(R0.copy())
+        if omp:
+            L.qobref_gemm_sp_dense_omp(a.real, a.imag, m, n, P(cp), P(rv), P(nz), P(B), k, b.real, b.imag, P(R), 4)
+        else:
+            L.qobref_gemm_sp_dense(a.real, a.imag, m, n, P(cp), P(rv), P(nz), P(B), k, b.real, b.imag, P(R))
+        assert rel(R, want) < 1e-14
+        outs.append(R)
+    assert np.array_equal(outs[0], outs[1])
+    # right: R (r x n) = a B M + b R, B (r x m)
+    r = 11
+    B = np.asfortranarray(g.standard_normal((r, m)) + 1j * g.standard_normal((r, m)))
+    R0 = np.full((r, n), np.nan, complex) if b == 0 else g.standard_normal((r, n)) + 1j * g.standard_normal((r, n))
+    want = a * (M.T @ B.T).T + (0 if b == 0 else b * R0)
+    outs = []
+    for omp in (0, 1):
+        R = np.asfortranarray(R0.copy())
+        if omp:
+            L.qobref_gemm_dense_sp_omp(a.real, a.imag, r, m, n, P(cp), P(rv), P(nz), P(B), b.real, b.imag, P(R), 4)
+        else:
+            L.qobref_gemm_dense_sp(a.real, a.imag, r, m, n, P(cp), P(rv), P(nz), P(B), b.real, b.imag, P(R))
+        assert rel(R, want) < 1e-14
+        outs.append(R)
+    assert np.array_equal(outs[0], outs[1])
+
+
 def _lp_case(g):
     """A LazyProduct of a LazyTensor, a dense operator, a LazySum and a
     sparse operator on 2 x 3 sites (all 6 x 6), factor 0.5 - 0.25i."""
