@@ -34,7 +34,7 @@ extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_r
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
 extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order;
-extern int g_sell_align;
+extern int g_sell_align, g_csc_waves, g_csc_sweep;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -267,9 +267,11 @@ static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
 // column, neighbours by increasing degree) minimises that distance.  Returns
 // an empty vector (natural order) when the natural order already keeps the
 // distance within 64 columns, when the reordering does not at least halve it,
-// or when X^T X would have more than 2^26 entries to visit.
-static std::vector<int32_t> csc_column_order(const RowLists &Xcols, int64_t xrows) {
+// or when X^T X would have more than 2^26 entries to visit.  *sweep_bw gets
+// that distance for the order chosen (-1 when the graph was not built).
+static std::vector<int32_t> csc_column_order(const RowLists &Xcols, int64_t xrows, int64_t *sweep_bw) {
     const int64_t n = Xcols.nr;
+    *sweep_bw = 0;
     if (n < 2 || xrows < 1) return {};
     std::vector<int32_t> rp((size_t)xrows + 1, 0);  // rows of X as lists of columns
     for (int32_t r : Xcols.idx) rp[(size_t)r + 1]++;
@@ -278,7 +280,10 @@ static std::vector<int32_t> csc_column_order(const RowLists &Xcols, int64_t xrow
         visits += (int64_t)rp[r + 1] * rp[r + 1];
         rp[r + 1] += rp[r];
     }
-    if (visits > (1ll << 26)) return {};
+    if (visits > (1ll << 26)) {
+        *sweep_bw = -1;
+        return {};
+    }
     std::vector<int32_t> rc(Xcols.idx.size()), cur(rp.begin(), rp.end() - 1);
     for (int64_t c = 0; c < n; ++c)
         for (int32_t p = Xcols.ptr[c]; p < Xcols.ptr[c + 1]; ++p) rc[cur[Xcols.idx[p]]++] = (int32_t)c;
@@ -299,6 +304,7 @@ static std::vector<int32_t> csc_column_order(const RowLists &Xcols, int64_t xrow
         }
         ap[c + 1] = (int64_t)adj.size();
     }
+    *sweep_bw = bw_nat;
     if (bw_nat <= 64) return {};
     auto deg = [&](int64_t c) { return ap[c + 1] - ap[c]; };
     // breadth-first levels from s inside its component: returns the last level
@@ -368,6 +374,7 @@ static std::vector<int32_t> csc_column_order(const RowLists &Xcols, int64_t xrow
     for (int64_t c = 0; c < n; ++c)
         for (int64_t e = ap[c]; e < ap[c + 1]; ++e) bw = std::max<int64_t>(bw, std::abs(pos[c] - pos[adj[e]]));
     if (2 * bw > bw_nat) return {};
+    *sweep_bw = bw;
     return order;
 }
 
@@ -381,7 +388,9 @@ static qob_status build_csc(const RowLists &Xcols, int64_t xrows, qob_dtype dt, 
     C.dev.colptr = (const int32_t *)C.ptr.p;
     C.dev.row = (const int32_t *)C.row.p;
     C.dev.val = C.val.p;
-    const std::vector<int32_t> ord = csc_column_order(Xcols, xrows);
+    int64_t bw = -1;
+    const std::vector<int32_t> ord = csc_column_order(Xcols, xrows, &bw);
+    C.dev.sweep_bw = (int32_t)std::min<int64_t>(bw, 1 << 30);
     if (!ord.empty()) {
         QOB_TRY(C.ord.upload(ord.data(), ord.size() * 4));
         C.dev.corder = (const int32_t *)C.ord.p;
@@ -2163,7 +2172,8 @@ static int *tune_slot(const char *name) {
                {"qubit_split", &g_qubit_split},   {"modek_nth", &g_modek_nth},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
-               {"sell_align", &g_sell_align}};
+               {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
+               {"csc_sweep", &g_csc_sweep}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2182,6 +2192,9 @@ qob_status qob_tune_set(const char *name, int64_t value) {
         return fail(QOB_EARG, "modek_nth must be 64, 128, 256 or 512");
     if (!strcmp(name, "sell_unr") && value != 2 && value != 4 && value != 8)
         return fail(QOB_EARG, "sell_unr must be 2, 4 or 8");
+    if (!strcmp(name, "csc_sweep") && value > 2) return fail(QOB_EARG, "csc_sweep must be 0, 1 or 2");
+    if (!strcmp(name, "csc_waves") && value != 1 && value != 2 && value != 4)
+        return fail(QOB_EARG, "csc_waves must be 1, 2 or 4");
     if (!strcmp(name, "csc_rows") && value != 16 && value != 32 && value != 64)
         return fail(QOB_EARG, "csc_rows must be 16, 32 or 64");
     if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)
@@ -2212,7 +2225,8 @@ qob_status qob_csc_column_order(int64_t n, int64_t xrows, const int32_t *colptr,
     X.idx.assign(rowind, rowind + X.ptr[n]);
     for (int32_t r : X.idx)
         if (r < 0 || r >= xrows) return fail(QOB_EARG, "row index %d out of range", r);
-    const std::vector<int32_t> ord = csc_column_order(X, xrows);
+    int64_t bw = -1;
+    const std::vector<int32_t> ord = csc_column_order(X, xrows, &bw);
     *used = ord.empty() ? 0 : 1;
     for (int64_t c = 0; c < n; ++c) order[c] = ord.empty() ? (int32_t)c : ord[c];
     return QOB_OK;

@@ -139,24 +139,34 @@ __global__ void __launch_bounds__(256) k_sell_left_rr(int nslices, int64_t m, co
 }
 
 // ---- right: CSC column gather, lanes along rows of B/R.
-// Logical block = (row block rb, column group cg), column group fastest, so
-// one XCD sweeps a stripe of rows across the columns.  Column slot k of the
-// sweep is output column corder[k] when a traversal order is given: an order
-// that puts columns sharing a column of B next to each other makes the
-// second read of that column an L2 hit.
-template <typename T, int RB>
-__global__ void __launch_bounds__(RB) k_csc_right(int64_t rows, int64_t n, const int32_t *__restrict__ colptr,
-                                                  const int32_t *__restrict__ rowind, const T *__restrict__ val,
-                                                  const T *__restrict__ B, T *__restrict__ R, double2 alpha_,
-                                                  double2 beta_, int beta_zero, uint32_t ncg, int cc, int remap,
-                                                  const int32_t *__restrict__ corder) {
+// Column slot k of the sweep is output column corder[k] when a traversal
+// order is given: an order that puts columns sharing a column of B next to
+// each other makes the second read of that column an L2 hit.  Two sweeps:
+//   colmajor = 0: logical block = (row block rb, column group cg), column
+//     group fastest: each XCD sweeps a stripe of rows across all columns,
+//     so B is reused within the stripe however far apart the columns are;
+//   colmajor = 1: row block fastest: each XCD takes a contiguous range of
+//     column slots and finishes every row block of a slot before the next,
+//     so reads and writes stream whole columns.  B is reused in L2 only when
+//     columns sharing a column of B are a few slots apart.
+// With W > 1 a workgroup holds W waves of RB = 64 rows each; wave w takes
+// the next cc column slots after wave w - 1 (a column group is W * cc slots).
+template <typename T, int RB, int W = 1>
+__global__ void __launch_bounds__(RB * W) k_csc_right(int64_t rows, int64_t n, const int32_t *__restrict__ colptr,
+                                                      const int32_t *__restrict__ rowind, const T *__restrict__ val,
+                                                      const T *__restrict__ B, T *__restrict__ R, double2 alpha_,
+                                                      double2 beta_, int beta_zero, uint32_t ncg, int cc, int remap,
+                                                      const int32_t *__restrict__ corder, int colmajor,
+                                                      uint32_t nrb) {
     const uint32_t G = gridDim.x;
     const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
-    const uint32_t cg = lb % ncg, rb = lb / ncg;
-    const int64_t i = (int64_t)rb * RB + threadIdx.x;
+    const uint32_t cg = colmajor ? lb / nrb : lb % ncg, rb = colmajor ? lb % nrb : lb / ncg;
+    // wave index made explicitly wave-uniform so the column's CSC entries stay scalar loads
+    const int wv = (W > 1) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / RB)) : 0;
+    const int64_t i = (int64_t)rb * RB + (W > 1 ? threadIdx.x % RB : threadIdx.x);
     const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
     const bool ok = i < rows;
-    const int64_t c0 = (int64_t)cg * cc;
+    const int64_t c0 = ((int64_t)cg * W + wv) * cc;
     const int64_t c1 = (c0 + cc < n) ? c0 + cc : n;
     for (int64_t k = c0; k < c1; ++k) {
         const int64_t c = corder ? (int64_t)corder[k] : k;
@@ -273,6 +283,8 @@ int g_csc_cc = 2;
 int g_csc_remap = 1;
 int g_csc_rows = 64;  // 64: one column per wave step (k_csc_right); 16 or 32: short stripes
 int g_csc_order = 1;  // 1: k_csc_right sweeps columns in the table's traversal order when it has one
+int g_csc_waves = 1;  // waves per k_csc_right workgroup when csc_rb is 64 (1, 2 or 4)
+int g_csc_sweep = 2;  // k_csc_right sweep: 0 row stripes, 1 column-major, 2 column-major when sweep_bw <= 8
 int g_nt_store = 1;
 
 template <typename T>
@@ -367,22 +379,33 @@ hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_
     }
     const int rb = g_csc_rb;
     const int cc = g_csc_cc;
+    const int nw = (rb == 64 && (g_csc_waves == 2 || g_csc_waves == 4)) ? g_csc_waves : 1;
     const uint32_t nrb = (uint32_t)((rows + rb - 1) / rb);
-    const uint32_t ncg = (uint32_t)((X.n + cc - 1) / cc);
+    const uint32_t ncg = (uint32_t)((X.n + (int64_t)cc * nw - 1) / ((int64_t)cc * nw));
     const uint64_t G = (uint64_t)nrb * ncg;
     if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
-#define QOB_CSC(TT, RBV)                                                                                  \
-    hipLaunchKernelGGL((k_csc_right<TT, RBV>), dim3((uint32_t)G), dim3(RBV), 0, s, rows, X.n, X.colptr, X.row, \
-                       (const TT *)X.val, (const TT *)B, (TT *)R, alpha, beta, bz, ncg, cc, g_csc_remap, \
-                       g_csc_order ? X.corder : nullptr)
+    const int32_t *corder = g_csc_order ? X.corder : nullptr;
+    // the column-major sweep needs the reuse distance of the order swept
+    const int64_t bw = corder ? X.sweep_bw : (X.corder ? -1 : X.sweep_bw);
+    const int colmajor = g_csc_sweep == 1 || (g_csc_sweep == 2 && bw >= 0 && bw <= 8);
+#define QOB_CSC(TT, RBV, WV)                                                                                     \
+    hipLaunchKernelGGL((k_csc_right<TT, RBV, WV>), dim3((uint32_t)G), dim3(RBV * WV), 0, s, rows, X.n, X.colptr, \
+                       X.row, (const TT *)X.val, (const TT *)B, (TT *)R, alpha, beta, bz, ncg, cc, g_csc_remap,  \
+                       corder, colmajor, nrb)
     if (dt == QOB_C64) {
-        if (rb == 64) QOB_CSC(double2, 64);
-        else if (rb == 128) QOB_CSC(double2, 128);
-        else QOB_CSC(double2, 256);
+        if (rb == 64) {
+            if (nw == 4) QOB_CSC(double2, 64, 4);
+            else if (nw == 2) QOB_CSC(double2, 64, 2);
+            else QOB_CSC(double2, 64, 1);
+        } else if (rb == 128) QOB_CSC(double2, 128, 1);
+        else QOB_CSC(double2, 256, 1);
     } else {
-        if (rb == 64) QOB_CSC(float2, 64);
-        else if (rb == 128) QOB_CSC(float2, 128);
-        else QOB_CSC(float2, 256);
+        if (rb == 64) {
+            if (nw == 4) QOB_CSC(float2, 64, 4);
+            else if (nw == 2) QOB_CSC(float2, 64, 2);
+            else QOB_CSC(float2, 64, 1);
+        } else if (rb == 128) QOB_CSC(float2, 128, 1);
+        else QOB_CSC(float2, 256, 1);
     }
 #undef QOB_CSC
     return hipGetLastError();

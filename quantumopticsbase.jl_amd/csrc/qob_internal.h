@@ -172,6 +172,7 @@ struct CscDev {
     const int32_t *colptr = nullptr, *row = nullptr;
     const void *val = nullptr;
     const int32_t *corder = nullptr;  // traversal order of the columns (nullptr: natural)
+    int32_t sweep_bw = -1;  // largest distance in that order between columns sharing a row (-1: unknown)
 };
 // R (m x cols) = alpha * X * B + beta * R, B (n x cols)
 hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols,

@@ -89,9 +89,9 @@ def test_fock_spin_sparse_left_right(fock_spin):
 
 
 def test_fock_spin_right_column_order_bit_exact(fock_spin):
-    """The right product sweeps columns in reverse Cuthill-McKee order; each
-    output column is still summed in the same order, so the result equals
-    the natural sweep bit for bit."""
+    """The right product sweeps columns in reverse Cuthill-McKee order, by
+    row stripes or column-major; each output column is still summed in the
+    same order, so every sweep equals the natural one bit for bit."""
     import ctypes as C
     from qobhip import _lib
     spec, rho = fock_spin
@@ -100,15 +100,20 @@ def test_fock_spin_right_column_order_bit_exact(fock_spin):
     outs = []
     v = C.c_int64()
     _lib.call("qob_tune_get", b"csc_order", C.byref(v))
+    w = C.c_int64()
+    _lib.call("qob_tune_get", b"csc_sweep", C.byref(w))
     try:
-        for flag in (1, 0):
+        for flag, sweep in ((1, 2), (1, 1), (1, 0), (0, 0)):
             _lib.call("qob_tune_set", b"csc_order", flag)
+            _lib.call("qob_tune_set", b"csc_sweep", sweep)
             out = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
             Q.mul_(out, rho, H, 0.5 - 0.25j, 0)
             outs.append(out.to_numpy())
     finally:
         _lib.call("qob_tune_set", b"csc_order", v.value)
-    assert np.array_equal(outs[0], outs[1])
+        _lib.call("qob_tune_set", b"csc_sweep", w.value)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
 
 
 def test_fock_spin_left_aligned_table_bit_exact(fock_spin):

@@ -331,13 +331,15 @@ def test_fused_equals_chain(dtype):
     assert rel(outs[0], outs[1]) < tol
 
 
-@pytest.mark.parametrize("rows_tunable", [16, 32, 64])
+@pytest.mark.parametrize("rows_tunable,waves,sweep", [(16, 1, 0), (32, 1, 0), (64, 1, 0), (64, 1, 1), (64, 2, 0),
+                                                      (64, 4, 1), (64, 4, 2)])
 @pytest.mark.parametrize("cc", [1, 3])
 @pytest.mark.parametrize("adj", [False, True])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
-def test_sparse_right_stripes(rows_tunable, cc, adj, dtype, tol):
-    """rho * SparseOperator through every stripe height of the CSC right kernel
-    (rows >= 32, so the wide kernels run), beta = 0 over NaN and beta != 0."""
+def test_sparse_right_stripes(rows_tunable, waves, sweep, cc, adj, dtype, tol):
+    """rho * SparseOperator through every stripe height, waves per workgroup
+    and sweep of the CSC right kernel (rows >= 32, so the wide kernels run),
+    beta = 0 over NaN and beta != 0."""
     from qobhip import _lib
     import ctypes as C
     g = np.random.default_rng(31 + rows_tunable + cc)
@@ -351,13 +353,15 @@ def test_sparse_right_stripes(rows_tunable, cc, adj, dtype, tol):
     mo, no = oop.shape
     B = g.standard_normal((rows, mo)) + 1j * g.standard_normal((rows, mo))
     saved = {}
-    for k in ("csc_rows", "csc_cc"):
+    for k in ("csc_rows", "csc_cc", "csc_waves", "csc_sweep"):
         v = C.c_int64()
         _lib.call("qob_tune_get", k.encode(), C.byref(v))
         saved[k] = v.value
     try:
         _lib.call("qob_tune_set", b"csc_rows", rows_tunable)
         _lib.call("qob_tune_set", b"csc_cc", cc)
+        _lib.call("qob_tune_set", b"csc_waves", waves)
+        _lib.call("qob_tune_set", b"csc_sweep", sweep)
         for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
             R0 = np.full((rows, no), np.nan, complex) if beta == 0 else g.standard_normal((rows, no)) + 0j
             want = np.zeros((rows, no), complex) if beta == 0 else R0.copy()
