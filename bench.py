@@ -247,7 +247,11 @@ def run_sparse6400(args):
         "lazysum_form_left_ms": {**lz_ms, "fused_speedup": lz_ms["plain"] / lz_ms["fused"] if "plain" in lz_ms else None},
         "expect_rho_ms_host_clock": ex_ms,
     }
-    roof = {"bound": "hbm", "kernel": "k_sell_left_rr" if dom == "left" else "k_csc_right",
+    # the Hamiltonian's pattern splits into components of at most 4 rows, so
+    # with closed blocks enabled the left product runs k_sell_left_closed
+    from qobhip import tune as T
+    left_kernel = "k_sell_left_closed" if T.get("sell_closed") else "k_sell_left_rr"
+    roof = {"bound": "hbm", "kernel": left_kernel if dom == "left" else "k_csc_right",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_of(f"csr_{dom}"), "algorithmic_bytes_per_launch": bytes_per_launch}
     return base_result(args, applies_per_s, "applies/s", ms_per_step, "f64",

@@ -34,7 +34,7 @@ extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_r
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
 extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order;
-extern int g_sell_align, g_csc_waves, g_csc_sweep;
+extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -168,7 +168,7 @@ enum { V_N = 0, V_T = 1, V_CJ = 2, V_H = 3 };
 
 // --------------------------------------------------------------- sparse op
 struct SellTab {
-    DevBuf off, col, val;
+    DevBuf off, col, val, rows;
     SellDev dev;
 };
 struct CscTab {
@@ -205,7 +205,104 @@ struct SparseData {
 // empty marker (-1) and read nothing.
 int g_sell_align = 1;
 
+// Closed blocks.  For a square X whose pattern (rows i and k joined when
+// X[i,k] != 0, either way round) splits into connected components of at most
+// 256 rows, whole components are packed into blocks of 256 rows (components
+// in order of their first row, a new block when the next one does not fit;
+// rows ascending within a block; a block is 4 slices of 64 lanes).  Every
+// row of B that a block reads is then the row one of its own lanes owns:
+// each lane loads its own row of B once per column into the workgroup's LDS
+// and the entries read from there, so B is read exactly once.  Such
+// components are the blocks of a conserved quantity (the chains of a
+// Jaynes-Cummings-like exchange term, for example).  Returns false
+// (contiguous slices) for any other X.
+int g_sell_closed = 1;
+
+static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
+    constexpr int BLK = 256;
+    const int64_t m = X.nr;
+    if (m != X.nc || m < 2 || m >= (1ll << 31)) return false;
+    std::vector<int32_t> parent((size_t)m);
+    for (int64_t i = 0; i < m; ++i) parent[i] = (int32_t)i;
+    auto find = [&](int32_t a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    for (int64_t i = 0; i < m; ++i)
+        for (int32_t q = X.ptr[i]; q < X.ptr[i + 1]; ++q) {
+            int32_t a = find((int32_t)i), b = find(X.idx[q]);
+            if (a != b) parent[std::max(a, b)] = std::min(a, b);
+        }
+    std::vector<int32_t> size((size_t)m, 0);
+    for (int64_t i = 0; i < m; ++i) {
+        if (++size[find((int32_t)i)] > BLK) return false;
+    }
+    // components in order of their first (smallest) row; rows of each component ascending
+    std::vector<int32_t> start((size_t)m + 1, 0), cur, rows_of((size_t)m);
+    for (int64_t i = 0; i < m; ++i) start[find((int32_t)i) + 1]++;
+    for (int64_t i = 0; i < m; ++i) start[i + 1] += start[i];
+    cur.assign(start.begin(), start.end() - 1);
+    for (int64_t i = 0; i < m; ++i) rows_of[cur[find((int32_t)i)]++] = (int32_t)i;
+    std::vector<int32_t> lanes;  // block-major, BLK per block, -1 idle
+    int used = BLK;
+    for (int64_t r = 0; r < m; ++r) {
+        if (start[r + 1] == start[r]) continue;  // r is not a component root
+        const int sz = start[r + 1] - start[r];
+        if (used + sz > BLK) {
+            lanes.insert(lanes.end(), BLK, -1);
+            used = 0;
+        }
+        const size_t base = lanes.size() - BLK;
+        for (int k = 0; k < sz; ++k) lanes[base + used + k] = rows_of[start[r] + k];
+        used += sz;
+    }
+    const int nblocks = (int)(lanes.size() / BLK), nslices = nblocks * (BLK / 64);
+    for (int k = 0; k < nblocks; ++k) {
+        auto b = lanes.begin() + (int64_t)k * BLK, e = b + BLK;
+        std::sort(b, std::find(b, e, -1));
+    }
+    std::vector<int32_t> lane_of((size_t)m, -1), off(nslices + 1, 0);
+    for (size_t e = 0; e < lanes.size(); ++e)
+        if (lanes[e] >= 0) lane_of[lanes[e]] = (int32_t)(e % BLK);
+    for (int k = 0; k < nslices; ++k) {
+        int w = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int32_t r = lanes[(size_t)k * 64 + l];
+            if (r >= 0) w = std::max(w, X.ptr[r + 1] - X.ptr[r]);
+        }
+        if (w > 16) return false;  // the kernel keeps a row's entries in registers
+        off[k + 1] = off[k] + std::max(w, 1) * 64;
+    }
+    std::vector<int32_t> col(off[nslices], -1);
+    std::vector<cd> val(off[nslices], cd(0, 0));
+    for (int k = 0; k < nslices; ++k)
+        for (int l = 0; l < 64; ++l) {
+            const int32_t r = lanes[(size_t)k * 64 + l];
+            if (r < 0) continue;
+            int p = 0;
+            for (int32_t q = X.ptr[r]; q < X.ptr[r + 1]; ++q, ++p) {
+                const int64_t e = off[k] + (int64_t)p * 64 + l;
+                col[e] = lane_of[X.idx[q]];
+                val[e] = X.val[q];
+            }
+        }
+    if (S.off.upload(off.data(), off.size() * 4) != QOB_OK || S.col.upload(col.data(), col.size() * 4) != QOB_OK ||
+        upload_vals(S.val, val, dt) != QOB_OK || S.rows.upload(lanes.data(), lanes.size() * 4) != QOB_OK)
+        return false;
+    S.dev.m = m;
+    S.dev.n = X.nc;
+    S.dev.nslices = nslices;
+    S.dev.maxw = 0;
+    for (int k = 0; k < nslices; ++k) S.dev.maxw = std::max(S.dev.maxw, (off[k + 1] - off[k]) / 64);
+    S.dev.slice_off = (const int32_t *)S.off.p;
+    S.dev.col = (const int32_t *)S.col.p;
+    S.dev.val = S.val.p;
+    S.dev.rows = (const int32_t *)S.rows.p;
+    return true;
+}
+
 static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
+    if (g_sell_closed && build_sell_closed(X, dt, S)) return QOB_OK;
     const int64_t m = X.nr;
     const int nslices = (int)((m + 63) / 64);
     std::vector<int32_t> off(nslices + 1, 0);
@@ -2173,7 +2270,8 @@ static int *tune_slot(const char *name) {
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
-               {"csc_sweep", &g_csc_sweep}};
+               {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed},
+               {"sell_cjchunk", &g_sell_cjchunk}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2200,11 +2298,11 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)
         return fail(QOB_EARG, "fused_gpw must be 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
-    if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align")) && value > 1)
+    if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed")) && value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
         return fail(QOB_EARG, "qubit_nth must be 256, 512 or 1024");
-    if ((!strcmp(name, "csc_cc") || !strcmp(name, "sell_jchunk")) && value < 1)
+    if ((!strcmp(name, "csc_cc") || !strcmp(name, "sell_jchunk") || !strcmp(name, "sell_cjchunk")) && value < 1)
         return fail(QOB_EARG, "%s must be >= 1", name);
     *p = (int)value;
     return QOB_OK;

@@ -138,6 +138,67 @@ __global__ void __launch_bounds__(256) k_sell_left_rr(int nslices, int64_t m, co
     }
 }
 
+// ---- left, closed blocks: lane t of workgroup k owns row rows[256k + t]
+// of R and of B (-1: idle).  The block's rows are whole connected components
+// of the pattern of X, so each entry of X in the block names a lane of the
+// same block (col = that lane).  Per batch of UNR columns every lane loads
+// its own row of B once into the workgroup's LDS and the entries read from
+// there: each element of B is loaded once, with no re-reads through L2.
+template <typename T, int MAXW, int UNR>
+__global__ void __launch_bounds__(256) k_sell_left_closed(int nslices, const int32_t *__restrict__ slice_off,
+                                                          const int32_t *__restrict__ rows,
+                                                          const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                          const T *__restrict__ B, int64_t ldb, int64_t cols,
+                                                          T *__restrict__ R, int64_t ldr, double2 alpha_,
+                                                          double2 beta_, int beta_zero, uint32_t nsg, int jchunk) {
+    __shared__ T sb[UNR][256];
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = xcd_remap(blockIdx.x, G);
+    const uint32_t sg = lb % nsg, cg = lb / nsg;
+    const int t = threadIdx.x;
+    const int slice = (int)(sg * 4 + (t >> 6));  // nslices is a multiple of 4: every lane has a slice
+    const int r = t & 63;
+    const int32_t row = rows[(int64_t)sg * 256 + t];
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+    const int32_t o0 = slice_off[slice];
+    const int w = (slice_off[slice + 1] - o0) >> 6;
+    int32_t c[MAXW];
+    T v[MAXW];
+#pragma unroll
+    for (int p = 0; p < MAXW; ++p) {
+        c[p] = (p < w) ? col[o0 + p * 64 + r] : -1;
+        v[p] = (p < w) ? val[o0 + p * 64 + r] : czero<T>();
+    }
+    const int64_t j0 = (int64_t)cg * jchunk;
+    const int64_t j1 = (j0 + jchunk < cols) ? j0 + jchunk : cols;
+    for (int64_t j = j0; j < j1; j += UNR) {
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+            sb[u][t] = (row >= 0 && j + u < j1) ? B[(int64_t)row + (j + u) * ldb] : czero<T>();
+        __syncthreads();
+        T acc[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) acc[u] = czero<T>();
+#pragma unroll
+        for (int p = 0; p < MAXW; ++p) {
+            if (c[p] >= 0) {
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) acc[u] = cfma(v[p], sb[u][c[p]], acc[u]);
+            }
+        }
+        __syncthreads();  // every lane has read this batch before the next one is written
+        if (row >= 0) {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                if (j + u < j1) {
+                    T *o = R + row + (j + u) * ldr;
+                    st_out(o, axpby(alpha, acc[u], beta, o, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
+                }
+            }
+        }
+    }
+}
+
 // ---- right: CSC column gather, lanes along rows of B/R.
 // Column slot k of the sweep is output column corder[k] when a traversal
 // order is given: an order that puts columns sharing a column of B next to
@@ -284,6 +345,7 @@ int g_csc_remap = 1;
 int g_csc_rows = 64;  // 64: one column per wave step (k_csc_right); 16 or 32: short stripes
 int g_csc_order = 1;  // 1: k_csc_right sweeps columns in the table's traversal order when it has one
 int g_csc_waves = 1;  // waves per k_csc_right workgroup when csc_rb is 64 (1, 2 or 4)
+int g_sell_cjchunk = 4;  // columns per workgroup of the closed-block left kernel
 int g_csc_sweep = 2;  // k_csc_right sweep: 0 row stripes, 1 column-major, 2 column-major when sweep_bw <= 8
 int g_nt_store = 1;
 
@@ -306,10 +368,41 @@ static void sell_rr(const SellDev &X, const void *B, int64_t cols, void *R, doub
 #undef QOB_RR
 }
 
+template <typename T>
+static void sell_closed(const SellDev &X, const void *B, int64_t cols, void *R, double2 alpha, double2 beta, int bz,
+                        hipStream_t s, uint32_t nsg, uint32_t G, int jchunk) {
+#define QOB_CL(MW, U)                                                                                           \
+    hipLaunchKernelGGL((k_sell_left_closed<T, MW, U>), dim3(G), dim3(256), 0, s, X.nslices, X.slice_off, X.rows, \
+                       X.col, (const T *)X.val, (const T *)B, X.n, cols, (T *)R, X.m, alpha, beta, bz, nsg, jchunk)
+#define QOB_CL_U(MW)                         \
+    do {                                     \
+        if (jchunk == 1) QOB_CL(MW, 1);      \
+        else if (jchunk == 2) QOB_CL(MW, 2); \
+        else QOB_CL(MW, 4);                  \
+    } while (0)
+    if (X.maxw <= 2) QOB_CL_U(2);
+    else if (X.maxw <= 4) QOB_CL_U(4);
+    else if (X.maxw <= 8) QOB_CL_U(8);
+    else QOB_CL_U(16);
+#undef QOB_CL_U
+#undef QOB_CL
+}
+
 hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols, void *R,
                             double2 alpha, double2 beta, hipStream_t s) {
     if (X.m == 0 || cols == 0) return hipSuccess;
     const int bz = (beta.x == 0.0 && beta.y == 0.0) | (g_nt_store << 1);
+    if (X.rows) {  // closed blocks (built only for square X, at most 16 entries per row)
+        if (X.maxw > 16 || X.nslices % 4) return hipErrorInvalidValue;
+        const uint32_t nsg = (uint32_t)(X.nslices / 4);
+        const int jchunk = (int)std::max<int64_t>(1, std::min<int64_t>(g_sell_cjchunk, cols));
+        const uint64_t ncg = (uint64_t)((cols + jchunk - 1) / jchunk);
+        const uint64_t G = (uint64_t)nsg * ncg;
+        if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
+        if (dt == QOB_C64) sell_closed<double2>(X, B, cols, R, alpha, beta, bz, s, nsg, (uint32_t)G, jchunk);
+        else sell_closed<float2>(X, B, cols, R, alpha, beta, bz, s, nsg, (uint32_t)G, jchunk);
+        return hipGetLastError();
+    }
     if (g_sell_variant == 1 && X.maxw <= 16) {
         const uint32_t nsg = (uint32_t)((X.nslices + 3) / 4);
         const int jchunk = (int)std::max<int64_t>(1, std::min<int64_t>(g_sell_jchunk, cols));

@@ -117,8 +117,9 @@ def test_fock_spin_right_column_order_bit_exact(fock_spin):
 
 
 def test_fock_spin_left_aligned_table_bit_exact(fock_spin):
-    """Diagonal-aligned and packed slot tables of the left kernel sum each
-    row in the same (column) order: equal results bit for bit."""
+    """Closed-slice, diagonal-aligned and packed slot tables of the left
+    kernel sum each row in the same (column) order: equal results bit for
+    bit."""
     import ctypes as C
     from qobhip import _lib
     spec, rho = fock_spin
@@ -126,8 +127,11 @@ def test_fock_spin_left_aligned_table_bit_exact(fock_spin):
     outs = []
     v = C.c_int64()
     _lib.call("qob_tune_get", b"sell_align", C.byref(v))
+    w = C.c_int64()
+    _lib.call("qob_tune_get", b"sell_closed", C.byref(w))
     try:
-        for flag in (1, 0):
+        for closed, flag in ((1, 1), (0, 1), (0, 0)):
+            _lib.call("qob_tune_set", b"sell_closed", closed)
             _lib.call("qob_tune_set", b"sell_align", flag)
             H = Q.SparseOperator(spec.basis, spec.basis, spec.H)
             out = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
@@ -135,7 +139,9 @@ def test_fock_spin_left_aligned_table_bit_exact(fock_spin):
             outs.append(out.to_numpy())
     finally:
         _lib.call("qob_tune_set", b"sell_align", v.value)
-    assert np.array_equal(outs[0], outs[1])
+        _lib.call("qob_tune_set", b"sell_closed", w.value)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
 
 
 def test_fock_spin_lazy_form_equals_sparse(fock_spin):

@@ -466,6 +466,80 @@ def test_sparse_left_diagonal_aligned(adj, dtype, tol):
     assert np.array_equal(outs[0], outs[2])
 
 
+def _block_matrix(g, sizes, n_extra_isolated=0):
+    """Square sparse matrix whose pattern splits into connected blocks of the
+    given sizes (each block a random connected pattern), rows and columns
+    relabelled by one random permutation, plus isolated empty rows."""
+    blocks = []
+    for k in sizes:
+        M = np.zeros((k, k), complex)
+        for i in range(k):  # a path keeps the block connected
+            M[i, i] = g.standard_normal() + 1j * g.standard_normal()
+            if i + 1 < k:
+                M[i, i + 1] = g.standard_normal() + 1j * g.standard_normal()
+        extra = g.random((k, k)) < min(0.3, 3.0 / k)
+        M[extra] = g.standard_normal(extra.sum()) + 1j * g.standard_normal(extra.sum())
+        blocks.append(sp.csr_matrix(M))
+    if n_extra_isolated:
+        blocks.append(sp.csr_matrix((n_extra_isolated, n_extra_isolated), dtype=complex))
+    X = sp.block_diag(blocks, format="csr")
+    p = g.permutation(X.shape[0])
+    X = sp.csc_matrix(X[p][:, p])
+    X.sort_indices()
+    return X
+
+
+@pytest.mark.parametrize("sizes", [[4, 1, 3, 2, 4] * 40, [64, 1, 63, 30, 40, 2], [256, 3, 200, 60] * 2,
+                                   [5] * 30 + [300]])
+@pytest.mark.parametrize("adj", [False, True])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_sparse_left_closed_slices(sizes, adj, dtype, tol):
+    """X * B, X * psi and psi' * X for block-structured X: blocks of at most
+    256 rows (and at most 16 entries per row) are packed whole into 256-row
+    workgroup blocks and read B once through LDS; a block of 300 rows makes
+    the table fall back to contiguous slices.  Against the oracle with beta = 0 over
+    NaN and beta != 0, and bit for bit against the contiguous-slice table."""
+    import ctypes as C
+    from qobhip import _lib
+    g = np.random.default_rng(sum(sizes) + 7 * len(sizes))
+    X = _block_matrix(g, sizes, n_extra_isolated=5)
+    n, cols = X.shape[0], 11
+    oop = O.OSparse.from_scipy(X)
+    if adj:
+        oop = O.dagger(oop)
+    B = g.standard_normal((n, cols)) + 1j * g.standard_normal((n, cols))
+    v = C.c_int64()
+    _lib.call("qob_tune_get", b"sell_closed", C.byref(v))
+    outs = []
+    try:
+        for closed in (1, 0):
+            _lib.call("qob_tune_set", b"sell_closed", closed)
+            qop = to_q(oop)   # tables are built on first use, under this setting
+            for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
+                R0 = np.full((n, cols), np.nan, complex) if beta == 0 else g.standard_normal((n, cols)) + 0j
+                want = np.zeros((n, cols), complex) if beta == 0 else R0.copy()
+                O.mul_op_left(want, oop, B, alpha, beta)
+                R = Q.Operator(gb(n), gb(cols), R0, dtype)
+                Q.mul_(R, qop, Q.Operator(gb(n), gb(cols), B, dtype), alpha, beta)
+                assert rel(R.to_numpy(), want) < tol
+                if beta == 0:
+                    outs.append(R.to_numpy())
+            psi = B[:, 0].copy()
+            want = np.zeros(n, complex)
+            O.mul_ket(want, oop, psi, 0.7, 0.0)
+            r = Q.Ket(gb(n), np.full(n, np.nan, complex), dtype)
+            Q.mul_(r, qop, Q.Ket(gb(n), psi, dtype), 0.7, 0.0)
+            assert rel(r.to_numpy(), want) < tol
+            want = np.zeros(n, complex)
+            O.mul_bra(want, psi, oop, 1.0, 0.0)
+            r = Q.Bra(gb(n), np.zeros(n, complex), dtype)
+            Q.mul_(r, Q.Bra(gb(n), psi, dtype), qop)
+            assert rel(r.to_numpy(), want) < tol
+    finally:
+        _lib.call("qob_tune_set", b"sell_closed", v.value)
+    assert np.array_equal(outs[0], outs[1])
+
+
 def _q_lazyproduct(lp):
     B = comp((2, 3))
     ops = []
