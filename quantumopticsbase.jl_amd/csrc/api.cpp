@@ -34,7 +34,7 @@ extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_r
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
 extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order;
-extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk;
+extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel, g_csc_gremap, g_sell_cremap;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -172,7 +172,7 @@ struct SellTab {
     SellDev dev;
 };
 struct CscTab {
-    DevBuf ptr, row, val, ord;
+    DevBuf ptr, row, val, ord, gint, gcoef;
     CscDev dev;
 };
 
@@ -475,6 +475,91 @@ static std::vector<int32_t> csc_column_order(const RowLists &Xcols, int64_t xrow
     return order;
 }
 
+// Component groups for the right product R = B X.  For a square X whose
+// pattern (output column c joined with every column k of B it reads) splits
+// into connected components of at most 8 columns, each component becomes a
+// group: its output columns (ascending), the columns of B they read
+// (ascending), a mask of the stored entries and their values.  One wave per
+// (group, 64-row block) loads each column of B the group reads once and
+// forms every output column of the group from registers, so a column of B
+// is read once per row block instead of once per output column that names
+// it.  Entries of an output column are summed in ascending input column,
+// the CSC order, so results equal the CSC kernel bit for bit.  Groups are
+// listed by their first column.  Returns false for any other X.
+int g_csc_groups = 1;
+
+static bool build_csc_groups(const RowLists &Xcols, int64_t xrows, qob_dtype dt, CscTab &C) {
+    const int64_t n = Xcols.nr;
+    if (n != xrows || n < 2 || n >= (1ll << 31)) return false;
+    std::vector<int32_t> parent((size_t)n);
+    for (int64_t i = 0; i < n; ++i) parent[i] = (int32_t)i;
+    auto find = [&](int32_t a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    for (int64_t c = 0; c < n; ++c)
+        for (int32_t p = Xcols.ptr[c]; p < Xcols.ptr[c + 1]; ++p) {
+            int32_t a = find((int32_t)c), b = find(Xcols.idx[p]);
+            if (a != b) parent[std::max(a, b)] = std::min(a, b);
+        }
+    std::vector<int32_t> size((size_t)n, 0);
+    int smax = 0;
+    for (int64_t i = 0; i < n; ++i) smax = std::max(smax, ++size[find((int32_t)i)]);
+    if (smax > 8) return false;
+    const int GM = smax <= 2 ? 2 : smax <= 4 ? 4 : 8, NI = 4 + 2 * GM;
+    std::vector<int32_t> start((size_t)n + 1, 0), cur, members((size_t)n);
+    for (int64_t i = 0; i < n; ++i) start[find((int32_t)i) + 1]++;
+    for (int64_t i = 0; i < n; ++i) start[i + 1] += start[i];
+    cur.assign(start.begin(), start.end() - 1);
+    for (int64_t i = 0; i < n; ++i) members[cur[find((int32_t)i)]++] = (int32_t)i;
+    std::vector<int32_t> gint;
+    std::vector<cd> gcoef;
+    std::vector<int32_t> slot((size_t)n, -1);
+    int ngroups = 0;
+    for (int64_t r = 0; r < n; ++r) {
+        const int sz = start[r + 1] - start[r];
+        if (sz == 0) continue;
+        const int32_t *mem = members.data() + start[r];
+        // inputs: members some output of the group reads, ascending
+        std::vector<int32_t> in;
+        for (int k = 0; k < sz; ++k) slot[mem[k]] = -1;
+        for (int k = 0; k < sz; ++k)
+            for (int32_t p = Xcols.ptr[mem[k]]; p < Xcols.ptr[mem[k] + 1]; ++p) slot[Xcols.idx[p]] = 0;
+        for (int k = 0; k < sz; ++k)
+            if (slot[mem[k]] == 0) {
+                slot[mem[k]] = (int32_t)in.size();
+                in.push_back(mem[k]);
+            }
+        const size_t gi = gint.size();
+        gint.resize(gi + NI, 0);
+        gcoef.resize((size_t)(ngroups + 1) * GM * GM, cd(0, 0));
+        gint[gi] = sz;
+        gint[gi + 1] = (int32_t)in.size();
+        uint64_t mask = 0;
+        for (int o = 0; o < sz; ++o) {
+            gint[gi + 4 + o] = mem[o];
+            for (int32_t p = Xcols.ptr[mem[o]]; p < Xcols.ptr[mem[o] + 1]; ++p) {
+                const int q = slot[Xcols.idx[p]];
+                const uint64_t bit = 1ull << (o * GM + q);
+                if (mask & bit) return false;  // repeated entry: leave it to the CSC kernel
+                mask |= bit;
+                gcoef[(size_t)ngroups * GM * GM + (size_t)o * GM + q] = Xcols.val[p];
+            }
+        }
+        for (size_t q = 0; q < in.size(); ++q) gint[gi + 4 + GM + q] = in[q];
+        gint[gi + 2] = (int32_t)(uint32_t)(mask & 0xffffffffu);
+        gint[gi + 3] = (int32_t)(uint32_t)(mask >> 32);
+        ++ngroups;
+    }
+    if (C.gint.upload(gint.data(), gint.size() * 4) != QOB_OK || upload_vals(C.gcoef, gcoef, dt) != QOB_OK)
+        return false;
+    C.dev.gint = (const int32_t *)C.gint.p;
+    C.dev.gcoef = C.gcoef.p;
+    C.dev.ngroups = ngroups;
+    C.dev.gmax = GM;
+    return true;
+}
+
 static qob_status build_csc(const RowLists &Xcols, int64_t xrows, qob_dtype dt, CscTab &C) {
     // Xcols: columns of X as lists (i.e. rows of X^T)
     QOB_TRY(C.ptr.upload(Xcols.ptr.data(), Xcols.ptr.size() * 4));
@@ -492,6 +577,7 @@ static qob_status build_csc(const RowLists &Xcols, int64_t xrows, qob_dtype dt, 
         QOB_TRY(C.ord.upload(ord.data(), ord.size() * 4));
         C.dev.corder = (const int32_t *)C.ord.p;
     }
+    if (g_csc_groups) build_csc_groups(Xcols, xrows, dt, C);
     return QOB_OK;
 }
 
@@ -2271,7 +2357,9 @@ static int *tune_slot(const char *name) {
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
                {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed},
-               {"sell_cjchunk", &g_sell_cjchunk}};
+               {"sell_cjchunk", &g_sell_cjchunk}, {"csc_groups", &g_csc_groups},
+               {"csc_gkernel", &g_csc_gkernel},   {"csc_gremap", &g_csc_gremap},
+               {"sell_cremap", &g_sell_cremap}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2298,7 +2386,10 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)
         return fail(QOB_EARG, "fused_gpw must be 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
-    if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed")) && value > 1)
+    if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
+         !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
+         !strcmp(name, "sell_cremap")) &&
+        value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
         return fail(QOB_EARG, "qubit_nth must be 256, 512 or 1024");

@@ -11,9 +11,13 @@
 // beta scaling (the reference's separate fill!/rmul! pass):
 //   left : lanes own rows of R and walk the rows of X, held as sliced ELL
 //          (64-row slices, one wavefront per slice, coalesced index loads);
+//          when the pattern of X splits into small connected components,
+//          whole components share a workgroup and read B once through LDS;
 //   right: lanes own rows of R (contiguous), one output column per step,
 //          the column's CSC entries are wave-uniform (scalar loads) and each
-//          gathers a contiguous run of a column of B.
+//          gathers a contiguous run of a column of B; when the pattern splits
+//          into components of at most 8 columns, one wave forms a whole
+//          component from one load of each column of B it reads.
 // The adjoint variants use the same kernels on the conjugate-transposed
 // tables built at upload time.
 #include "qob_internal.h"
@@ -150,10 +154,11 @@ __global__ void __launch_bounds__(256) k_sell_left_closed(int nslices, const int
                                                           const int32_t *__restrict__ col, const T *__restrict__ val,
                                                           const T *__restrict__ B, int64_t ldb, int64_t cols,
                                                           T *__restrict__ R, int64_t ldr, double2 alpha_,
-                                                          double2 beta_, int beta_zero, uint32_t nsg, int jchunk) {
+                                                          double2 beta_, int beta_zero, uint32_t nsg, int jchunk,
+                                                          int remap) {
     __shared__ T sb[UNR][256];
     const uint32_t G = gridDim.x;
-    const uint32_t lb = xcd_remap(blockIdx.x, G);
+    const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
     const uint32_t sg = lb % nsg, cg = lb / nsg;
     const int t = threadIdx.x;
     const int slice = (int)(sg * 4 + (t >> 6));  // nslices is a multiple of 4: every lane has a slice
@@ -239,6 +244,45 @@ __global__ void __launch_bounds__(RB * W) k_csc_right(int64_t rows, int64_t n, c
             }
             T *o = R + i + c * rows;
             st_out(o, axpby(alpha, acc, beta, o, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
+        }
+    }
+}
+
+// ---- right, component groups: one wave per (group, 64-row block), row
+// block fastest so each XCD streams a contiguous range of groups.  The
+// group's record is wave-uniform: the columns of B it reads are loaded once
+// into registers and every output column is formed from them, its stored
+// entries (mask bits) in ascending input column.
+template <typename T, int GM>
+__global__ void __launch_bounds__(64) k_csc_right_groups(int64_t rows, const int32_t *__restrict__ gint,
+                                                         const T *__restrict__ gcoef, const T *__restrict__ B,
+                                                         T *__restrict__ R, double2 alpha_, double2 beta_,
+                                                         int beta_zero, uint32_t nrb, int remap) {
+    constexpr int NI = 4 + 2 * GM;
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
+    const uint32_t g = lb / nrb, rb = lb % nrb;
+    const int32_t *gi = gint + (int64_t)g * NI;
+    const T *gc = gcoef + (int64_t)g * GM * GM;
+    const int nout = gi[0], nin = gi[1];
+    const uint64_t mask = (uint64_t)(uint32_t)gi[2] | ((uint64_t)(uint32_t)gi[3] << 32);
+    const int64_t i = (int64_t)rb * 64 + threadIdx.x;
+    const bool ok = i < rows;
+    const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
+    T b[GM];
+#pragma unroll
+    for (int q = 0; q < GM; ++q) b[q] = (q < nin && ok) ? B[i + (int64_t)gi[4 + GM + q] * rows] : czero<T>();
+#pragma unroll
+    for (int o = 0; o < GM; ++o) {
+        if (o < nout) {
+            T acc = czero<T>();
+#pragma unroll
+            for (int q = 0; q < GM; ++q)
+                if ((mask >> (o * GM + q)) & 1ull) acc = cfma(gc[o * GM + q], b[q], acc);
+            if (ok) {
+                T *out = R + i + (int64_t)gi[4 + o] * rows;
+                st_out(out, axpby(alpha, acc, beta, out, (beta_zero & 1) != 0), (beta_zero & 2) != 0);
+            }
         }
     }
 }
@@ -346,6 +390,13 @@ int g_csc_rows = 64;  // 64: one column per wave step (k_csc_right); 16 or 32: s
 int g_csc_order = 1;  // 1: k_csc_right sweeps columns in the table's traversal order when it has one
 int g_csc_waves = 1;  // waves per k_csc_right workgroup when csc_rb is 64 (1, 2 or 4)
 int g_sell_cjchunk = 4;  // columns per workgroup of the closed-block left kernel
+int g_csc_gkernel = 1;   // 1: the right product uses the table's component groups when it has them
+// XCD remap of the group and closed-block kernels.  Both read each element
+// of rho once, so there is no L2 reuse to keep on one XCD: the group kernel
+// streams better with the plain round-robin dispatch (neighbouring blocks on
+// different XCDs); the closed-block kernel measured faster remapped.
+int g_csc_gremap = 0;
+int g_sell_cremap = 1;
 int g_csc_sweep = 2;  // k_csc_right sweep: 0 row stripes, 1 column-major, 2 column-major when sweep_bw <= 8
 int g_nt_store = 1;
 
@@ -373,7 +424,8 @@ static void sell_closed(const SellDev &X, const void *B, int64_t cols, void *R, 
                         hipStream_t s, uint32_t nsg, uint32_t G, int jchunk) {
 #define QOB_CL(MW, U)                                                                                           \
     hipLaunchKernelGGL((k_sell_left_closed<T, MW, U>), dim3(G), dim3(256), 0, s, X.nslices, X.slice_off, X.rows, \
-                       X.col, (const T *)X.val, (const T *)B, X.n, cols, (T *)R, X.m, alpha, beta, bz, nsg, jchunk)
+                       X.col, (const T *)X.val, (const T *)B, X.n, cols, (T *)R, X.m, alpha, beta, bz, nsg, jchunk, \
+                       g_sell_cremap)
 #define QOB_CL_U(MW)                         \
     do {                                     \
         if (jchunk == 1) QOB_CL(MW, 1);      \
@@ -449,6 +501,25 @@ hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_
         else
             hipLaunchKernelGGL(k_csc_right_thin<float2>, dim3(g), dim3(256), 0, s, rows, X.n, X.colptr, X.row,
                                (const float2 *)X.val, (const float2 *)B, (float2 *)R, alpha, beta, bz);
+        return hipGetLastError();
+    }
+    if (X.gint && g_csc_gkernel) {
+        const uint64_t nrb = (uint64_t)((rows + 63) / 64);
+        const uint64_t G = nrb * (uint64_t)X.ngroups;
+        if (G > 0x7fffffffull || X.ngroups <= 0) return hipErrorInvalidConfiguration;
+#define QOB_CG(TT, GMV)                                                                                   \
+    hipLaunchKernelGGL((k_csc_right_groups<TT, GMV>), dim3((uint32_t)G), dim3(64), 0, s, rows, X.gint,   \
+                       (const TT *)X.gcoef, (const TT *)B, (TT *)R, alpha, beta, bz, (uint32_t)nrb, g_csc_gremap)
+        if (dt == QOB_C64) {
+            if (X.gmax == 2) QOB_CG(double2, 2);
+            else if (X.gmax == 4) QOB_CG(double2, 4);
+            else QOB_CG(double2, 8);
+        } else {
+            if (X.gmax == 2) QOB_CG(float2, 2);
+            else if (X.gmax == 4) QOB_CG(float2, 4);
+            else QOB_CG(float2, 8);
+        }
+#undef QOB_CG
         return hipGetLastError();
     }
     if (g_csc_rows == 16 || g_csc_rows == 32) {

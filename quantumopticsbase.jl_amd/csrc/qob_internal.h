@@ -178,6 +178,12 @@ struct CscDev {
     const void *val = nullptr;
     const int32_t *corder = nullptr;  // traversal order of the columns (nullptr: natural)
     int32_t sweep_bw = -1;  // largest distance in that order between columns sharing a row (-1: unknown)
+    // Component groups (gint != nullptr): ngroups fixed-size records of
+    // gmax (2, 4 or 8) columns: ints [nout, nin, mask_lo, mask_hi,
+    // out[gmax], in[gmax]] and gmax x gmax coefficients (row = output).
+    const int32_t *gint = nullptr;
+    const void *gcoef = nullptr;
+    int32_t ngroups = 0, gmax = 0;
 };
 // R (m x cols) = alpha * X * B + beta * R, B (n x cols)
 hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols,

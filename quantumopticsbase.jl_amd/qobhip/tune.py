@@ -11,10 +11,12 @@ of libqob_hip.so (qob_tune_set / qob_tune_get):
   sparse products   sell_variant, sell_cj, sell_jchunk, sell_unr, csc_rows,
                     csc_rb, csc_cc, csc_remap, csc_order (column sweep order),
                     csc_sweep (row stripes / column-major / by reuse
-                    distance), csc_waves, sell_cjchunk (columns per
-                    closed-block workgroup), nt_store; sell_align
-                    (diagonal-aligned slots) and sell_closed (closed
-                    blocks) apply to tables built after they are set
+                    distance), csc_waves, csc_gkernel / csc_gremap
+                    (component-group right kernel, its XCD remap),
+                    sell_cjchunk / sell_cremap (closed-block left kernel),
+                    nt_store; sell_align (diagonal-aligned slots),
+                    sell_closed (closed blocks) and csc_groups (component
+                    groups) apply to tables built after they are set
 
 ``tune`` runs a coordinate search: starting from the current values it
 tries every value of one tunable with the others fixed, keeps the fastest,
@@ -43,9 +45,10 @@ SPACES: Dict[str, Dict[str, List[int]]] = {
     "qubit": {"qubit_tbits": [10, 11, 12, 13], "qubit_nth": [256, 512, 1024], "qubit_grid": [0, 768, 1024, 2048],
               "qubit_remap": [0, 1], "qubit_split": [0, 1]},
     "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16, 32],
-                    "sell_unr": [2, 4, 8], "sell_cjchunk": [4, 2, 8]},
+                    "sell_unr": [2, 4, 8], "sell_cjchunk": [4, 2, 8], "sell_cremap": [1, 0]},
     "sparse_right": {"csc_rows": [16, 32, 64], "csc_rb": [64, 128, 256], "csc_cc": [1, 2, 4, 8], "csc_remap": [0, 1],
-                     "csc_order": [1, 0], "csc_sweep": [2, 0, 1], "csc_waves": [1, 2, 4]},
+                     "csc_order": [1, 0], "csc_sweep": [2, 0, 1], "csc_waves": [1, 2, 4], "csc_gkernel": [1, 0],
+                     "csc_gremap": [0, 1]},
 }
 
 

@@ -89,9 +89,10 @@ def test_fock_spin_sparse_left_right(fock_spin):
 
 
 def test_fock_spin_right_column_order_bit_exact(fock_spin):
-    """The right product sweeps columns in reverse Cuthill-McKee order, by
-    row stripes or column-major; each output column is still summed in the
-    same order, so every sweep equals the natural one bit for bit."""
+    """The right product runs on component groups, or sweeps columns in
+    reverse Cuthill-McKee order by row stripes or column-major; each output
+    column is still summed in the same order, so every form equals the
+    natural sweep bit for bit."""
     import ctypes as C
     from qobhip import _lib
     spec, rho = fock_spin
@@ -100,10 +101,12 @@ def test_fock_spin_right_column_order_bit_exact(fock_spin):
     outs = []
     v = C.c_int64()
     _lib.call("qob_tune_get", b"csc_order", C.byref(v))
-    w = C.c_int64()
+    w, gk = C.c_int64(), C.c_int64()
     _lib.call("qob_tune_get", b"csc_sweep", C.byref(w))
+    _lib.call("qob_tune_get", b"csc_gkernel", C.byref(gk))
     try:
-        for flag, sweep in ((1, 2), (1, 1), (1, 0), (0, 0)):
+        for gkern, flag, sweep in ((1, 1, 2), (0, 1, 2), (0, 1, 1), (0, 1, 0), (0, 0, 0)):
+            _lib.call("qob_tune_set", b"csc_gkernel", gkern)
             _lib.call("qob_tune_set", b"csc_order", flag)
             _lib.call("qob_tune_set", b"csc_sweep", sweep)
             out = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
@@ -112,14 +115,14 @@ def test_fock_spin_right_column_order_bit_exact(fock_spin):
     finally:
         _lib.call("qob_tune_set", b"csc_order", v.value)
         _lib.call("qob_tune_set", b"csc_sweep", w.value)
+        _lib.call("qob_tune_set", b"csc_gkernel", gk.value)
     for o in outs[1:]:
         assert np.array_equal(outs[0], o)
 
 
 def test_fock_spin_left_aligned_table_bit_exact(fock_spin):
-    """Closed-slice, diagonal-aligned and packed slot tables of the left
-    kernel sum each row in the same (column) order: equal results bit for
-    bit."""
+    """Closed-block, diagonal-aligned and packed tables of the left kernel
+    sum each row in the same (column) order: equal results bit for bit."""
     import ctypes as C
     from qobhip import _lib
     spec, rho = fock_spin

@@ -489,7 +489,7 @@ def _block_matrix(g, sizes, n_extra_isolated=0):
     return X
 
 
-@pytest.mark.parametrize("sizes", [[4, 1, 3, 2, 4] * 40, [64, 1, 63, 30, 40, 2], [256, 3, 200, 60] * 2,
+@pytest.mark.parametrize("sizes", [[4, 1, 3, 2, 4] * 40, [2, 1] * 90, [64, 1, 63, 30, 40, 2], [256, 3, 200, 60] * 2,
                                    [5] * 30 + [300]])
 @pytest.mark.parametrize("adj", [False, True])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
@@ -497,8 +497,9 @@ def test_sparse_left_closed_slices(sizes, adj, dtype, tol):
     """X * B, X * psi and psi' * X for block-structured X: blocks of at most
     256 rows (and at most 16 entries per row) are packed whole into 256-row
     workgroup blocks and read B once through LDS; a block of 300 rows makes
-    the table fall back to contiguous slices.  Against the oracle with beta = 0 over
-    NaN and beta != 0, and bit for bit against the contiguous-slice table."""
+    the table fall back to contiguous slices.  Each layout is built in turn
+    and checked against the oracle with beta = 0 over NaN and beta != 0,
+    and bit for bit against the contiguous-slice table."""
     import ctypes as C
     from qobhip import _lib
     g = np.random.default_rng(sum(sizes) + 7 * len(sizes))
@@ -514,7 +515,7 @@ def test_sparse_left_closed_slices(sizes, adj, dtype, tol):
     try:
         for closed in (1, 0):
             _lib.call("qob_tune_set", b"sell_closed", closed)
-            qop = to_q(oop)   # tables are built on first use, under this setting
+            qop = to_q(oop)   # tables are built on first use, under these settings
             for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
                 R0 = np.full((n, cols), np.nan, complex) if beta == 0 else g.standard_normal((n, cols)) + 0j
                 want = np.zeros((n, cols), complex) if beta == 0 else R0.copy()
@@ -537,6 +538,46 @@ def test_sparse_left_closed_slices(sizes, adj, dtype, tol):
             assert rel(r.to_numpy(), want) < tol
     finally:
         _lib.call("qob_tune_set", b"sell_closed", v.value)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
+@pytest.mark.parametrize("sizes", [[1, 2, 3, 4] * 50, [8, 7, 1, 5] * 20, [2] * 60, [9] + [3] * 40])
+@pytest.mark.parametrize("adj", [False, True])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_sparse_right_component_groups(sizes, adj, dtype, tol):
+    """rho * X for block-structured X: components of at most 8 columns run
+    as groups (one load of each column of rho per row block); a 9-column
+    component leaves the table without groups.  Against the oracle with
+    beta = 0 over NaN and beta != 0, and bit for bit against the CSC
+    kernel."""
+    import ctypes as C
+    from qobhip import _lib
+    g = np.random.default_rng(sum(sizes) + 3 * len(sizes))
+    X = _block_matrix(g, sizes, n_extra_isolated=3)
+    n, rows = X.shape[0], 70
+    oop = O.OSparse.from_scipy(X)
+    if adj:
+        oop = O.dagger(oop)
+    qop = to_q(oop)
+    B = g.standard_normal((rows, n)) + 1j * g.standard_normal((rows, n))
+    v = C.c_int64()
+    _lib.call("qob_tune_get", b"csc_gkernel", C.byref(v))
+    outs = []
+    try:
+        for gk in (1, 0):
+            _lib.call("qob_tune_set", b"csc_gkernel", gk)
+            for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
+                R0 = np.full((rows, n), np.nan, complex) if beta == 0 else g.standard_normal((rows, n)) + 0j
+                want = np.zeros((rows, n), complex) if beta == 0 else R0.copy()
+                O.mul_op_right(want, B, oop, alpha, beta)
+                R = Q.Operator(gb(rows), gb(n), R0, dtype)
+                Q.mul_(R, Q.Operator(gb(rows), gb(n), B, dtype), qop, alpha, beta)
+                assert rel(R.to_numpy(), want) < tol
+                if beta == 0:
+                    outs.append(R.to_numpy())
+    finally:
+        _lib.call("qob_tune_set", b"csc_gkernel", v.value)
     assert np.array_equal(outs[0], outs[1])
 
 
