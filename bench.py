@@ -193,6 +193,7 @@ def run_sparse6400(args):
     # (16 N^2) + write out (16 N^2) + CSR/CSC (nnz*(16+4) + (N+1)*4).
     ev = EventTimer(ctx.h, stream)
     reps = max(20, min(200, args.steps))
+    from qobhip import tune as T
     ms_left = ev(left, reps)
     ms_right = ev(right, reps)
     # the same bytes moved by the library's streaming copy kernel (read rho,
@@ -208,6 +209,8 @@ def run_sparse6400(args):
 
     # the same Hamiltonian as a LazySum of LazyTensors: fused (one gather) vs
     # the plain per-term path
+    # ("fused" runs it as its summed sparse matrix on rho, "gather" evaluates
+    # every term per element in one launch)
     lz = W.fock_fock_spin_lazy(spec)
     outZ = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
     lz_ms = {}
@@ -218,6 +221,13 @@ def run_sparse6400(args):
         ctx.synchronize()
         lz_ms[name] = ev(f, 10)
     ctx.set_mode(Q.QOB_MODE_FUSED)
+    T.set_("ls_materialize", 0)
+    lzg = W.fock_fock_spin_lazy(spec)
+    f = lambda: Q.mul_(outZ, lzg, rho, 1.0, 0.0)
+    f()
+    ctx.synchronize()
+    lz_ms["gather"] = ev(f, 10)
+    T.set_("ls_materialize", 1)
 
     # expect(H, rho) = tr(H rho): one pass over the nonzeros (fused) against
     # the plain form (apply into a temporary, then trace)
@@ -249,7 +259,6 @@ def run_sparse6400(args):
     }
     # the Hamiltonian's pattern splits into components of at most 4 rows, so
     # with closed blocks enabled the left product runs k_sell_left_closed
-    from qobhip import tune as T
     left_kernel = "k_sell_left_closed" if T.get("sell_closed") else "k_sell_left_rr"
     roof = {"bound": "hbm", "kernel": left_kernel if dom == "left" else "k_csc_right",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1721,6 +1722,128 @@ static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *ou
 // LazySum (src/operators_lazysum.jl:189-238).  Fused mode groups the terms:
 // all pure-sparse LazyTensors become one gather launch, all SparseOperators
 // are summed into one sparse matrix; the remaining terms follow with beta=1.
+// Materialised LazySum.  On a density matrix (eight or more columns, or rows
+// for the right application) an all-sparse LazySum is cheaper as its summed
+// sparse matrix: the table is read once per launch and the sparse kernels
+// (closed blocks, component groups) then read rho once, where the gather
+// decodes every term's multi-index per element.  The sum of the all-sparse
+// LazyTensor terms (coefficient and factor folded in; sites without a factor
+// are identities truncated to min(left, right) dimension, as in the gather)
+// and the SparseOperator terms is expanded on the host, duplicate columns of
+// a row summed, and cached per coefficient version.  Not used with
+// device-held coefficients (their values change on the device) or above
+// 2^22 stored entries.
+int g_ls_materialize = 1;
+
+static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const std::vector<cd> &lcoef,
+                           const std::vector<size_t> &sp_terms, std::shared_ptr<SparseData> &out) {
+    std::lock_guard<std::mutex> g(op.mu);
+    const std::string key = "mat" + std::to_string(op.version);
+    auto it = op.sums.find(key);
+    if (it != op.sums.end()) {
+        out = it->second;
+        return out != nullptr;
+    }
+    out = nullptr;
+    const std::vector<int64_t> &shl = lts[0]->shl, &shr = lts[0]->shr;
+    const int ns = (int)shl.size();
+    // stored entries: per term, product over sites of the factor's nnz (or the identity's size)
+    double est = 0;
+    std::vector<std::vector<RowLists>> store(lts.size());
+    std::vector<TermSpec> specs;
+    for (size_t t = 0; t < lts.size(); ++t) {
+        const OpNode &lt = *lts[t];
+        if (lt.shl != shl || lt.shr != shr) {
+            op.sums.emplace(key, nullptr);
+            return false;
+        }
+        TermSpec ts;
+        ts.coef = lcoef[t] * lt.factor;
+        for (size_t i = 0; i < lt.fop.size(); ++i)
+            if (lt.fop[i]->kind != K_EYE) store[t].push_back(factor_rows(*lt.fop[i], false));
+        size_t k2 = 0;
+        for (size_t i = 0; i < lt.fop.size(); ++i) {
+            if (lt.fop[i]->kind == K_EYE) continue;
+            ts.sites.push_back(lt.fsite[i]);
+            ts.X.push_back(&store[t][k2++]);
+        }
+        double e = 1;
+        for (int site = 0; site < ns; ++site) {
+            auto f = std::find(ts.sites.begin(), ts.sites.end(), site);
+            e *= (f == ts.sites.end()) ? (double)std::min(shl[site], shr[site])
+                                       : (double)ts.X[f - ts.sites.begin()]->idx.size();
+        }
+        est += e;
+        specs.push_back(std::move(ts));
+    }
+    for (size_t t : sp_terms) est += (double)op.terms[t]->sp->csr.idx.size();
+    if (est > (double)(1 << 22)) {
+        op.sums.emplace(key, nullptr);
+        return false;
+    }
+    const int64_t m = op.m, n = op.n;
+    std::vector<int64_t> stl(ns, 1), str(ns, 1);
+    for (int site = 1; site < ns; ++site) {
+        stl[site] = stl[site - 1] * shl[site - 1];
+        str[site] = str[site - 1] * shr[site - 1];
+    }
+    std::vector<RowLists> sprows;
+    for (size_t t : sp_terms) sprows.push_back(op.terms[t]->sp->rows(op.terms[t]->var));
+    auto S = std::make_shared<SparseData>();
+    S->m = m;
+    S->n = n;
+    S->csr.nr = m;
+    S->csr.nc = n;
+    S->csr.ptr.assign((size_t)m + 1, 0);
+    std::vector<int64_t> dig(ns);
+    std::vector<std::pair<int64_t, cd>> row;
+    for (int64_t i = 0; i < m; ++i) {
+        row.clear();
+        int64_t rem = i;
+        for (int site = ns - 1; site >= 0; --site) {
+            dig[site] = rem / stl[site];
+            rem %= stl[site];
+        }
+        for (const TermSpec &ts : specs) {
+            // depth-first over sites: column offset and value so far
+            std::function<void(int, int64_t, cd)> rec = [&](int site, int64_t k, cd v) {
+                if (site == ns) {
+                    row.emplace_back(k, v);
+                    return;
+                }
+                auto f = std::find(ts.sites.begin(), ts.sites.end(), site);
+                if (f == ts.sites.end()) {
+                    if (dig[site] < shr[site]) rec(site + 1, k + dig[site] * str[site], v);
+                    return;
+                }
+                const RowLists &X = *ts.X[f - ts.sites.begin()];
+                for (int32_t p = X.ptr[dig[site]]; p < X.ptr[dig[site] + 1]; ++p)
+                    rec(site + 1, k + (int64_t)X.idx[p] * str[site], v * X.val[p]);
+            };
+            rec(0, 0, ts.coef);
+        }
+        for (size_t q = 0; q < sp_terms.size(); ++q) {
+            const RowLists &R = sprows[q];
+            const cd c = op.coef[sp_terms[q]];
+            for (int32_t p = R.ptr[i]; p < R.ptr[i + 1]; ++p) row.emplace_back(R.idx[p], c * R.val[p]);
+        }
+        std::stable_sort(row.begin(), row.end(),
+                         [](const std::pair<int64_t, cd> &a, const std::pair<int64_t, cd> &b) { return a.first < b.first; });
+        for (size_t q = 0; q < row.size();) {
+            int64_t k = row[q].first;
+            cd v = 0;
+            for (; q < row.size() && row[q].first == k; ++q) v += row[q].second;
+            S->csr.idx.push_back((int32_t)k);
+            S->csr.val.push_back(v);
+        }
+        S->csr.ptr[i + 1] = (int32_t)S->csr.idx.size();
+    }
+    S->csc = transpose_rows(S->csr, false);
+    op.sums.emplace(key, S);
+    out = S;
+    return true;
+}
+
 static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
     const int64_t dout = op_out_dim(op, ap.trans);
     if (op.terms.empty() || is_zero(alpha)) return apply_scale(ap, dout, out, beta);
@@ -1751,6 +1874,21 @@ static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *ou
         }
     }
     cd b = beta;
+    const int64_t nvec = ap.lead == 1 ? ap.trail : ap.lead;
+    if (!lts.empty() && g_ls_materialize && !op.dcoef_on && nvec >= 8 && op.m < (1ll << 31) && op.n < (1ll << 31)) {
+        std::shared_ptr<SparseData> SD;
+        if (ls_materialize(op, lts, lcoef, sp_terms, SD)) {
+            OpNode tmp;
+            tmp.kind = K_SPARSE;
+            tmp.m = op.m;
+            tmp.n = op.n;
+            tmp.sp = SD;
+            QOB_TRY(apply_sparse(ap, tmp, in, out, alpha, b));
+            b = cd(1, 0);
+            lts.clear();
+            sp_terms.clear();
+        }
+    }
     if (!lts.empty()) {
         GatherPlanDev P;
         QOB_TRY(lazy_plan(op, "ls" + std::to_string(op.version), lts, lcoef, ap.trans, ap.dt, P,
@@ -2359,7 +2497,7 @@ static int *tune_slot(const char *name) {
                {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed},
                {"sell_cjchunk", &g_sell_cjchunk}, {"csc_groups", &g_csc_groups},
                {"csc_gkernel", &g_csc_gkernel},   {"csc_gremap", &g_csc_gremap},
-               {"sell_cremap", &g_sell_cremap}};
+               {"sell_cremap", &g_sell_cremap},   {"ls_materialize", &g_ls_materialize}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2388,7 +2526,7 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
-         !strcmp(name, "sell_cremap")) &&
+         !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)

@@ -175,6 +175,74 @@ def test_lazysum(mode, alpha, beta):
         ctx.set_mode(Q.QOB_MODE_FUSED)
 
 
+def _materialize_case(g):
+    """LazySum on shl = (2, 3, 2) -> shr = (2, 4, 2): two all-sparse terms on
+    the same sites (their entries overlap and are summed), a term whose
+    unfactored middle site is a truncating identity (3 x 4), a SparseOperator
+    term, and a dense-factor term that stays outside the summed matrix."""
+    shl, shr = (2, 3, 2), (2, 4, 2)
+    ops = [rand_lt(g, shl, shr, (2, 3), ("sparse", "sparse")),
+           rand_lt(g, shl, shr, (2, 3), ("sparse", "sparse")),
+           rand_lt(g, shl, shr, (1,), ("sparse",)),
+           O.OSparse.from_scipy(rs(g, 12, 16, 0.2)),
+           rand_lt(g, shl, shr, (3,), ("dense",))]
+    return O.OLazySum(12, 16, [0.3, -1.1 + 0.2j, 0.7j, 0.5, -0.4], ops)
+
+
+@pytest.mark.parametrize("alpha,beta", AB)
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_lazysum_materialized_on_rho(alpha, beta, dtype, tol):
+    """With ten columns (left) or nine rows (right) an all-sparse LazySum runs
+    as its summed sparse matrix; against the oracle, and against the gather
+    (materialisation off) to rounding."""
+    import ctypes as C
+    from qobhip import _lib
+    g = np.random.default_rng(33)
+    ls = _materialize_case(g)
+    v = C.c_int64()
+    _lib.call("qob_tune_get", b"ls_materialize", C.byref(v))
+    got = {}
+    try:
+        for mat in (1, 0):
+            _lib.call("qob_tune_set", b"ls_materialize", mat)
+            q = to_q(ls)
+            bl, br = q.basis_l, q.basis_r
+            B = cg(np.random.default_rng(5), 16, 10)
+            R0 = np.full((12, 10), np.nan, complex) if beta == 0 else cg(np.random.default_rng(6), 12, 10)
+            want = np.zeros((12, 10), complex) if beta == 0 else R0.copy()
+            O.mul_op_left(want, ls, B, alpha, beta)
+            R = Q.Operator(bl, gb(10), R0, dtype)
+            Q.mul_(R, q, Q.Operator(br, gb(10), B, dtype), alpha, beta)
+            assert rel(R.to_numpy(), want) < tol, "left"
+            B2 = cg(np.random.default_rng(7), 9, 12)
+            R2 = np.full((9, 16), np.nan, complex) if beta == 0 else cg(np.random.default_rng(8), 9, 16)
+            want2 = np.zeros((9, 16), complex) if beta == 0 else R2.copy()
+            O.mul_op_right(want2, B2, ls, alpha, beta)
+            R = Q.Operator(gb(9), br, R2, dtype)
+            Q.mul_(R, Q.Operator(gb(9), bl, B2, dtype), q, alpha, beta)
+            assert rel(R.to_numpy(), want2) < tol, "right"
+            got[mat] = R.to_numpy()
+    finally:
+        _lib.call("qob_tune_set", b"ls_materialize", v.value)
+    assert rel(got[1], got[0]) < tol
+
+
+def test_lazysum_materialized_follows_set_factors():
+    """The summed matrix is rebuilt when the coefficients change on the host."""
+    g = np.random.default_rng(34)
+    ls = _materialize_case(g)
+    q = to_q(ls)
+    B = cg(g, 16, 10)
+    for factors in ([0.3, -1.1 + 0.2j, 0.7j, 0.5, -0.4], [1.0, 0.0, -2.0, 0.25j, 0.1]):
+        q.set_factors(factors)
+        ls2 = O.OLazySum(12, 16, factors, ls.operators)
+        want = np.zeros((12, 10), complex)
+        O.mul_op_left(want, ls2, B, 1.0, 0.0)
+        R = Q.Operator(q.basis_l, gb(10), np.zeros((12, 10), complex))
+        Q.mul_(R, q, Q.Operator(q.basis_r, gb(10), B))
+        assert rel(R.to_numpy(), want) < TOL64
+
+
 def test_lazysum_zero_and_nan():
     g = np.random.default_rng(31)
     B = comp((2, 3))
