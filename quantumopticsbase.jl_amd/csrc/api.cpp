@@ -1703,12 +1703,29 @@ static qob_status apply_qubit(const Apply &ap, OpNode &op, const void *in, void 
     return QOB_OK;
 }
 
+extern int g_ls_materialize;
+static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const std::vector<cd> &lcoef,
+                           const std::vector<size_t> &sp_terms, std::shared_ptr<SparseData> &out);
+
 static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
     const int64_t dout = op_out_dim(op, ap.trans);
     if (is_zero(alpha)) return apply_scale(ap, dout, out, beta);  // :540
     bool done = false;
     QOB_TRY(apply_qubit(ap, op, in, out, alpha, beta, done));
     if (done) return QOB_OK;
+    const int64_t nvec = ap.lead == 1 ? ap.trail : ap.lead;
+    if (lt_pure_sparse(op) && g_ls_materialize && nvec >= 8 && op.m < (1ll << 31) && op.n < (1ll << 31)) {
+        // on a density matrix: the tensor's sparse matrix (see ls_materialize)
+        std::shared_ptr<SparseData> SD;
+        if (ls_materialize(op, {&op}, {cd(1, 0)}, {}, SD)) {
+            OpNode tmp;
+            tmp.kind = K_SPARSE;
+            tmp.m = op.m;
+            tmp.n = op.n;
+            tmp.sp = SD;
+            return apply_sparse(ap, tmp, in, out, alpha, beta);
+        }
+    }
     if (lt_pure_sparse(op)) {
         // _mul_puresparse! -> _gemm_puresparse (:729-751)
         GatherPlanDev P;
@@ -1719,9 +1736,6 @@ static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *ou
     return apply_lt_chain(ap, op, in, out, alpha, beta);
 }
 
-// LazySum (src/operators_lazysum.jl:189-238).  Fused mode groups the terms:
-// all pure-sparse LazyTensors become one gather launch, all SparseOperators
-// are summed into one sparse matrix; the remaining terms follow with beta=1.
 // Materialised LazySum.  On a density matrix (eight or more columns, or rows
 // for the right application) an all-sparse LazySum is cheaper as its summed
 // sparse matrix: the table is read once per launch and the sparse kernels
@@ -1844,6 +1858,12 @@ static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const s
     return true;
 }
 
+// LazySum (src/operators_lazysum.jl:189-238).  Fused mode groups the terms:
+// on a density matrix with host-held coefficients all pure-sparse
+// LazyTensors and SparseOperators become one summed sparse matrix;
+// otherwise the pure-sparse LazyTensors become one gather launch and the
+// SparseOperators one summed sparse matrix.  The remaining terms follow with
+// beta = 1.
 static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
     const int64_t dout = op_out_dim(op, ap.trans);
     if (op.terms.empty() || is_zero(alpha)) return apply_scale(ap, dout, out, beta);

@@ -123,6 +123,26 @@ def test_lazytensor(case, alpha, beta):
     run_all_paths(lt, to_q(lt), g, alpha, beta)
 
 
+@pytest.mark.parametrize("case", range(len(LT_CASES)))
+@pytest.mark.parametrize("materialize", [1, 0])
+def test_lazytensor_on_rho_materialized(case, materialize):
+    """Ten columns (left) and eleven rows (right): an all-sparse LazyTensor
+    runs as its sparse matrix (Eye factors and implicit isometries
+    included), or through the gather with materialisation off."""
+    import ctypes as C
+    from qobhip import _lib
+    shl, shr, idx, kinds = LT_CASES[case]
+    g = np.random.default_rng(50 + case)
+    lt = rand_lt(g, shl, shr, idx, kinds)
+    v = C.c_int64()
+    _lib.call("qob_tune_get", b"ls_materialize", C.byref(v))
+    try:
+        _lib.call("qob_tune_set", b"ls_materialize", materialize)
+        run_all_paths(lt, to_q(lt), g, 0.3 - 0.6j, 0.5 + 0.1j, ncols=10)
+    finally:
+        _lib.call("qob_tune_set", b"ls_materialize", v.value)
+
+
 @pytest.mark.parametrize("case", [0, 1, 4, 7])
 def test_lazytensor_c32(case):
     shl, shr, idx, kinds = LT_CASES[case]
