@@ -387,14 +387,18 @@ __global__ void __launch_bounds__(NTH) k_qubit_axpy(const QubitStateDev *__restr
 
 int g_qubit_nth = 512;
 int g_qubit_remap = 1;
-// 0: 3 workgroups per CU for ComplexF64 and 4 for ComplexF32. With 2^11-element tiles this keeps about
-// 2-3 MB of tiles in flight per XCD, inside its 4 MB L2, so partner tiles read by high-bit flips are
-// still resident; 5 per CU measured 5-15% slower on the 256-chain batch.
+// 0: one workgroup per item when the items fit in 5 workgroups per CU (many small states: no
+// persistent loop, every item starts at once); otherwise a persistent grid of 3 workgroups per CU
+// for ComplexF64 and 4 for ComplexF32. With 2^11-element tiles that keeps about 2-3 MB of tiles in
+// flight per XCD, inside its 4 MB L2, so partner tiles read by high-bit flips are still resident;
+// 5 per CU measured 5-15% slower on a batch of 256 chains of 8..20 sites.
 int g_qubit_grid = 0;
 
 static unsigned qubit_grid(qob_dtype dt, int nitems) {
     const int ncu = device_cu_count();
-    const int64_t g = g_qubit_grid > 0 ? g_qubit_grid : (int64_t)ncu * (dt == QOB_C64 ? 3 : 4);
+    const int64_t g = g_qubit_grid > 0           ? g_qubit_grid
+                      : nitems <= (int64_t)ncu * 5 ? nitems
+                                                   : (int64_t)ncu * (dt == QOB_C64 ? 3 : 4);
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, nitems));
 }
 

@@ -18,6 +18,7 @@ every ``mul_`` is one call into libqob_hip.so.  Python's ``!``-less naming:
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from fractions import Fraction
 from typing import Sequence
@@ -136,19 +137,28 @@ class Basis:
     def __init__(self, shape: Sequence[int], name: str):
         self.shape = tuple(int(s) for s in shape)
         self._name = name
+        self._len = math.prod(self.shape)
 
     def __len__(self) -> int:
-        return int(np.prod(self.shape, dtype=np.int64))
+        return self._len
 
     @property
     def bases(self):
         return (self,)
 
     def __eq__(self, other) -> bool:
-        return isinstance(other, Basis) and self._key() == other._key()
+        return self is other or (isinstance(other, Basis) and self._ckey() == other._ckey())
 
     def __hash__(self):
-        return hash(self._key())
+        return hash(self._ckey())
+
+    def _ckey(self):
+        # bases are immutable: the comparison key is built once (mul! checks
+        # bases on every call)
+        k = self.__dict__.get("_kc")
+        if k is None:
+            k = self._kc = self._key()
+        return k
 
     def _key(self):
         return (self._name, self.shape)
@@ -591,11 +601,16 @@ def dagger(x):
 
 
 # ---------------------------------------------------------------------- mul!
+_DT_CODE = {np.dtype(np.complex128): QOB_C64, np.dtype(np.complex64): QOB_C32}
+
+
 def _check_same_dtype(*xs):
-    dts = {x.dtype for x in xs}
-    if len(dts) != 1:
-        raise ArgumentError(f"mixed element types {dts}: states of one call share one type")
-    return _dt(dts.pop())
+    d0 = xs[0].dtype
+    for x in xs[1:]:
+        if x.dtype != d0:
+            raise ArgumentError(f"mixed element types {set(y.dtype for y in xs)}: states of one call share one type")
+    code = _DT_CODE.get(d0)
+    return _dt(d0) if code is None else code
 
 
 def mul_(result, a, b, alpha=1.0, beta=0.0):
@@ -667,8 +682,9 @@ def mul_(result, a, b, alpha=1.0, beta=0.0):
 def _bases(*pairs):
     # pairs of bases that must agree; lengths are checked by the library
     # (DimensionMismatch), basis identity here (IncompatibleBases).
-    for x, y in zip(pairs[::2], pairs[1::2]):
-        if len(x) == len(y) and x != y and not (_generic(x) or _generic(y)):
+    for i in range(0, len(pairs), 2):
+        x, y = pairs[i], pairs[i + 1]
+        if x is not y and len(x) == len(y) and x != y and not (_generic(x) or _generic(y)):
             raise IncompatibleBases(f"{x} != {y}")
 
 
