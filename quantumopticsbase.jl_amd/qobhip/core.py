@@ -779,9 +779,15 @@ def expect(op: AbstractOperator, state) -> complex:
     return complex(out.re, out.im)
 
 
-def variance(op: AbstractOperator, state: "Ket") -> complex:
+def variance(op: AbstractOperator, state) -> complex:
     """variance(op, ket) = <psi|op op|psi> - <psi|op|psi>^2 with x = op psi
-    formed once (src/operators.jl:139-143)."""
+    formed once (src/operators.jl:139-143); on a density matrix (the
+    operator form the docstring at src/operators.jl:132-137 names)
+    variance(op, rho) = tr(op op rho) - tr(op rho)^2 with x = op rho formed
+    once and both traces taken by expect (fused where the operator allows)."""
+    if isinstance(state, Operator):
+        x = op * state
+        return expect(op, x) - expect(op, state) ** 2
     if not isinstance(state, Ket):
         raise MethodError(f"variance not defined for {type(state).__name__}")
     if isinstance(op, Operator):

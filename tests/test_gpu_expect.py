@@ -122,3 +122,63 @@ def test_expect_follows_device_coefficients():
                          for _, i, fs in spec.terms])
         M = O.dense_of(Ho)
         assert abs(Q.expect(H, k) - _ev(M, psi)) < 1e-12 * max(1, abs(_ev(M, psi)))
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 1e-5)])
+def test_variance_rho_sparse_lazy_dense(dtype, tol):
+    """variance(op, rho) = tr(op op rho) - tr(op rho)^2 for a SparseOperator,
+    its LazySum form and a dense operator, against dense algebra."""
+    spec = W.fock_fock_spin(5, 1.5, 4)
+    n = spec.H.shape[0]
+    g = np.random.default_rng(8)
+    A = g.standard_normal((n, n)) + 1j * g.standard_normal((n, n))
+    rho = A @ A.conj().T
+    rho /= np.trace(rho)
+    if dtype == np.complex64:
+        rho = rho.astype(np.complex64).astype(np.complex128)
+    M = spec.H.toarray()
+    R = Q.Operator(spec.basis, spec.basis, rho, dtype)
+    want = np.trace(M @ M @ rho) - np.trace(M @ rho) ** 2
+    scale = max(1.0, abs(np.trace(M @ M @ rho)))
+    for op in (Q.SparseOperator(spec.basis, spec.basis, spec.H), W.fock_fock_spin_lazy(spec),
+               Q.DenseOperator(spec.basis, spec.basis, M, dtype)):
+        assert abs(Q.variance(op, R) - want) <= 10 * tol * scale, type(op).__name__
+
+
+def test_time_dependent_sum_expect_variance_follow_time():
+    """expect / variance of a TimeDependentSum are those of its static
+    operator at the current time (src/time_dependent_operator.jl:76-80), on
+    a Ket and on a density matrix."""
+    from qobhip.timedependent import TimeDependentSum
+    L = 6
+    g = np.random.default_rng(12)
+    terms = random_pauli_sum(g, L, 8)
+    Hq, _, B = build_pair(L, terms)
+    coefs = [lambda t, k=k: np.cos(0.3 * k * t) + 0.1j * k for k in range(len(Hq.operators))]
+    tds = TimeDependentSum(list(zip(coefs, Hq.operators)))
+    D = 2 ** L
+    psi = g.standard_normal(D) + 1j * g.standard_normal(D)
+    psi /= np.linalg.norm(psi)
+    rho = np.outer(psi, psi.conj())
+    k, R = Q.Ket(B, psi), Q.Operator(B, B, rho)
+    for t in (0.0, 0.7, 2.5):
+        tds.set_time_(t)
+        c = [f(t) for f in coefs]
+        M = sum(ci * O.dense_of(to_o) for ci, to_o in zip(c, _oracle_terms(Hq)))
+        assert abs(Q.expect(tds, k) - _ev(M, psi)) < 1e-12 * max(1, abs(_ev(M, psi)))
+        assert abs(Q.variance(tds, k) - _var(M, psi)) < 1e-11 * max(1, abs(_var(M, psi)))
+        want_e = np.trace(M @ rho)
+        want_v = np.trace(M @ M @ rho) - want_e ** 2
+        assert abs(Q.expect(tds, R) - want_e) < 1e-12 * max(1, abs(want_e))
+        assert abs(Q.variance(tds, R) - want_v) < 1e-11 * max(1, abs(want_v))
+
+
+def _oracle_terms(Hq):
+    """The oracle LazyTensors of a qobhip LazySum of qubit LazyTensors."""
+    out = []
+    for lt in Hq.operators:
+        sh = tuple(lt.basis_l.shape)
+        out.append(O.OLazyTensor(sh, sh, tuple(lt.indices),
+                                 tuple(O.OSparse.from_scipy(sp.csc_matrix(f.csc)) for f in lt.operators),
+                                 lt.factor))
+    return out
