@@ -291,6 +291,14 @@ function QuantumOpticsBase.variance(op::DevOp, state::Ket{B,<:DevVec{T}}) where 
     complex(out[].re, out[].im)
 end
 
+# variance(op, rho) = tr(op op rho) - tr(op rho)^2, op rho formed once on the
+# device (the operator form named by the docstring at src/operators.jl:132-137)
+function QuantumOpticsBase.variance(op::DevOp, state::Operator{B,B,<:DevMat{T}}) where {B,T<:CT}
+    x = Operator(state.basis_l, state.basis_r, similar(state.data))
+    mul!(x, op, state, true, false)
+    expect(op, x) - expect(op, state)^2
+end
+
 function QuantumOpticsBase.expect(op::DevOp, state::Ket{B,<:DevVec{T}}) where {B,T<:CT}
     out = Ref{QobC64}(QobC64(0, 0))
     check(ccall((:qob_expect_ket, libqob[]), Cint,
