@@ -669,6 +669,45 @@ def test_sparse_right_component_groups(sizes, adj, dtype, tol):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("kind", ["zero", "diagonal", "one_block"])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_sparse_rho_degenerate_components(kind, dtype, tol):
+    """Left and right products where the component layouts degenerate: an
+    all-zero matrix (singleton components, no entries: alpha * 0 + beta R,
+    NaN overwritten when beta = 0), a diagonal matrix (singletons with one
+    entry) and one dense 300-row block (too large for either layout)."""
+    g = np.random.default_rng(91)
+    n, k = 300, 40
+    if kind == "zero":
+        X = sp.csc_matrix((n, n), dtype=complex)
+    elif kind == "diagonal":
+        X = sp.diags(g.standard_normal(n) + 1j * g.standard_normal(n), format="csc")
+    else:
+        X = sp.csc_matrix(g.standard_normal((n, n)) * (g.random((n, n)) < 0.02) + 0j)
+        X = X + sp.diags(np.ones(n - 1), 1, format="csc")  # keeps it one component
+    oop = O.OSparse.from_scipy(sp.csc_matrix(X))
+    qop = to_q(oop)
+    for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
+        B = g.standard_normal((n, k)) + 1j * g.standard_normal((n, k))
+        R0 = np.full((n, k), np.nan, complex) if beta == 0 else g.standard_normal((n, k)) + 0j
+        want = np.zeros((n, k), complex) if beta == 0 else R0.copy()
+        O.mul_op_left(want, oop, B, alpha, beta)
+        R = Q.Operator(gb(n), gb(k), R0, dtype)
+        Q.mul_(R, qop, Q.Operator(gb(n), gb(k), B, dtype), alpha, beta)
+        got = R.to_numpy()
+        assert np.isfinite(got).all()
+        assert rel(got, want) < tol or np.abs(got - want).max() < tol
+        B2 = g.standard_normal((k, n)) + 1j * g.standard_normal((k, n))
+        R2 = np.full((k, n), np.nan, complex) if beta == 0 else g.standard_normal((k, n)) + 0j
+        want2 = np.zeros((k, n), complex) if beta == 0 else R2.copy()
+        O.mul_op_right(want2, B2, oop, alpha, beta)
+        R = Q.Operator(gb(k), gb(n), R2, dtype)
+        Q.mul_(R, Q.Operator(gb(k), gb(n), B2, dtype), qop, alpha, beta)
+        got = R.to_numpy()
+        assert np.isfinite(got).all()
+        assert rel(got, want2) < tol or np.abs(got - want2).max() < tol
+
+
 def _q_lazyproduct(lp):
     B = comp((2, 3))
     ops = []
