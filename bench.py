@@ -440,7 +440,7 @@ def run_dense8192(args):
             "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": 2 * N * N * np.dtype(dt).itemsize}
     config = {"workload": f"dense LazyTensor (16, 32, 16) left mul! on an 8192x8192 {args.dtype} density matrix",
               "N": N, "dims": dims, "kernel_ms": {"fused_left": ms_fused, "chain_left": ms_chain,
-                                                   "chain_right": ms_right}, **extra}
+                                                   "right_tiles": ms_right}, **extra}
     return base_result(args, args.steps / el, "applies/s", el * 1e3 / args.steps,
                        "f32" if dt == np.complex64 else "f64",
                        "synthetic (seeded Hermitian factors, Ginibre trace-1 rho)", config, roof, cpu)

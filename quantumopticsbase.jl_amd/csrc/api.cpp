@@ -34,7 +34,7 @@ namespace qob {
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
-extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order;
+extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles;
 extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel, g_csc_gremap, g_sell_cremap;
 
 // ------------------------------------------------------------------ errors
@@ -1646,6 +1646,56 @@ static qob_status apply_lt_chain(const Apply &ap, OpNode &op, const void *in, vo
             return QOB_OK;
         }
     }
+    // Right application on a column-major matrix with enough rows: the
+    // factors of the leading sites 0..k-1 (dimensions multiplying to P, with
+    // 16 P elements per tile fitting the kernel) run in one fused pass over
+    // tiles of 16 rows x P columns; the factors of the other sites run first
+    // through the chain into a temporary (factors on different sites
+    // commute).  Each pass reads and writes the matrix once.
+    if (g_fused_lt && g_fused_tiles && ap.lead >= 64 && ap.lead % 16 == 0 && ap.trail == 1 && !ops.empty() &&
+        sh_in == sh_out && Din < (1 << 30)) {
+        bool dense = true;
+        for (const ChainOp &c : ops)
+            if (c.f->kind != K_DENSE || c.d_in != c.d_out) dense = false;
+        const int64_t dmax = ap.dt == QOB_C64 ? 8192 : 16384;
+        int k = 0;
+        int64_t P = 1;
+        while (k < N && 16 * P * sh_in[k] <= dmax) P *= sh_in[k++];
+        std::vector<size_t> tile_i, rest_i;
+        for (size_t i = 0; i < ops.size(); ++i) (ops[i].site < k ? tile_i : rest_i).push_back(i);
+        if (dense && !tile_i.empty() && (int)tile_i.size() <= FUSED_MAXMODES) {
+            FusedModes FM;
+            FM.D = (int)(16 * P);
+            FM.nmodes = (int)tile_i.size();
+            for (size_t t = 0; t < tile_i.size(); ++t) {
+                const ChainOp &c = ops[tile_i[t]];
+                FM.n1[t] = (int)(16 * prod(sh_in, 0, c.site));
+                FM.d[t] = (int)c.d_in;
+                QOB_TRY(dense_dev(*c.f->de, c.f->var ^ (ap.trans ? V_T : V_N), ap.dt, FM.A[t]));
+            }
+            if (fused_supported(ap.dt, FM)) {
+                const void *src = in;
+                if (!rest_i.empty()) {
+                    std::vector<int64_t> cur = sh_in;
+                    const size_t el = (size_t)(Din * ap.lead);
+                    void *t1, *t2 = nullptr;
+                    QOB_TRY(ws_get(ap.ctx, ap.s, WS_TMP1, el * esize(ap.dt), t1));
+                    if (rest_i.size() > 1) QOB_TRY(ws_get(ap.ctx, ap.s, WS_TMP2, el * esize(ap.dt), t2));
+                    QOB_TRY(chain_step(ap, op, "lt", ops[rest_i[0]], cur, in, t1, cd(1, 0), cd(0, 0)));
+                    for (size_t i = 1; i < rest_i.size(); ++i) {
+                        QOB_TRY(chain_step(ap, op, "lt", ops[rest_i[i]], cur, t1, t2, cd(1, 0), cd(0, 0)));
+                        std::swap(t1, t2);
+                    }
+                    src = t1;
+                }
+                const int64_t nrb = ap.lead / 16;
+                hipError_t e = launch_lt_fused(ap.dt, FM, src, out, nrb * (Din / P), d2(a), d2(beta), ap.s,
+                                               ap.lead, nrb);
+                if (e != hipSuccess) return fail(QOB_EHIP, "fused LazyTensor tile launch: %s", hipGetErrorString(e));
+                return QOB_OK;
+            }
+        }
+    }
     if (ops.empty()) {
         // n_ops == 0: result = alpha * b + beta * result (:453-458)
         if (Din != Dout) return fail(QOB_EDIM, "LazyTensor without factors on unequal dimensions");
@@ -2517,7 +2567,8 @@ static int *tune_slot(const char *name) {
                {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed},
                {"sell_cjchunk", &g_sell_cjchunk}, {"csc_groups", &g_csc_groups},
                {"csc_gkernel", &g_csc_gkernel},   {"csc_gremap", &g_csc_gremap},
-               {"sell_cremap", &g_sell_cremap},   {"ls_materialize", &g_ls_materialize}};
+               {"sell_cremap", &g_sell_cremap},   {"ls_materialize", &g_ls_materialize},
+               {"fused_tiles", &g_fused_tiles}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2546,7 +2597,7 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
-         !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize")) &&
+         !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)

@@ -60,10 +60,30 @@ constexpr int NW = NTH / 64;
 // never overlap and no barrier is needed between compute and write-back.
 // MAXNTM: d/16 of the widest factor; EPT: column elements per lane; GPW:
 // fibre groups in flight per wave (more MFMA chains in flight, more registers).
-template <typename R, int MAXNTM, int EPT, int GPW>
+// TILE = false: vector c is the contiguous column in + c * D (left
+// application, Kets).  TILE = true (right application on a column-major
+// matrix with leading dimension T.ld): vector c is the tile of 16 rows
+// (row block c % T.nrb) by P = D / 16 consecutive columns (column block
+// c / T.nrb), element e at row 16 (c % T.nrb) + (e & 15) and column
+// (e >> 4) + P (c / T.nrb); the factors of the tile's modes act with the
+// row index innermost.
+struct FusedTile {
+    int64_t ld = 0;
+    int64_t nrb = 1;
+    int P = 0;
+};
+
+template <bool TILE>
+__device__ __forceinline__ int64_t fused_addr(const FusedTile &T, int D, int64_t c, int e) {
+    if (!TILE) return c * (int64_t)D + e;
+    const int64_t rb = c % T.nrb, q = c / T.nrb;
+    return rb * 16 + (e & 15) + T.ld * ((int64_t)(e >> 4) + (int64_t)T.P * q);
+}
+
+template <typename R, int MAXNTM, int EPT, int GPW, bool TILE>
 __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *__restrict__ in,
-                                                   cplx<R> *__restrict__ out, int64_t ncols, double2 alpha_,
-                                                   double2 beta_, int beta_zero) {
+                                                   cplx<R> *__restrict__ out, int64_t ncols, FusedTile TL,
+                                                   double2 alpha_, double2 beta_, int beta_zero) {
     using T = cplx<R>;
     using V = typename mf<R>::v4;
     extern __shared__ unsigned char lds_raw[];
@@ -93,11 +113,10 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
     T pre[EPT];
     int64_t col = blockIdx.x;
     if (col < ncols) {
-        const T *src = in + col * (int64_t)D;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
-            if (e < D) pre[k] = src[e];
+            if (e < D) pre[k] = in[fused_addr<TILE>(TL, D, col, e)];
         }
     }
     for (; col < ncols; col += gridDim.x) {
@@ -112,11 +131,10 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
         // issue the next column's loads now; they complete during the MFMAs
         const int64_t nxt = col + gridDim.x;
         if (nxt < ncols) {
-            const T *src = in + nxt * (int64_t)D;
 #pragma unroll
             for (int k = 0; k < EPT; ++k) {
                 const int e = tid + k * NTH;
-                if (e < D) pre[k] = src[e];
+                if (e < D) pre[k] = in[fused_addr<TILE>(TL, D, nxt, e)];
             }
         }
 
@@ -209,7 +227,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
                     // last mode: the tiles go from the accumulators straight
                     // to the output column (out = alpha y + beta out); the
                     // stores drain while the next column is being computed
-                    T *dst = out + col * (int64_t)D;
+                    T *dst = out;
 #pragma unroll
                     for (int q = 0; q < GPW; ++q) {
                         if (g0 + q * NW < ngroups) {
@@ -220,8 +238,9 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
                                     for (int r = 0; r < 4; ++r) {
                                         const int e = base[q] + n1 * (mt * 16 + mf<R>::row(lane, r));
                                         T v = cmul(alpha, mk((R)accr[q][mt][r], (R)acci[q][mt][r]));
-                                        if (!beta_zero) v = cfma(beta, dst[e], v);
-                                        st_stream(dst + e, v);
+                                        const int64_t ge = fused_addr<TILE>(TL, D, col, e);
+                                        if (!beta_zero) v = cfma(beta, dst[ge], v);
+                                        st_stream(dst + ge, v);
                                     }
                                 }
                             }
@@ -237,6 +256,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 }  // namespace
 
 int g_fused_gpw = 2;  // fibre groups in flight per wave: 1, 2 or 4
+int g_fused_tiles = 1;  // 1: right applications run the fused kernel on 16-row tiles
 
 // LDS bytes for a column of D elements (padded planes)
 static size_t fused_lds(qob_dtype dt, int D, int dmax) {
@@ -268,7 +288,15 @@ bool fused_supported(qob_dtype dt, const FusedModes &M) {
 }
 
 hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols,
-                           double2 alpha, double2 beta, hipStream_t s) {
+                           double2 alpha, double2 beta, hipStream_t s, int64_t tile_ld, int64_t tile_nrb) {
+    // tile_ld > 0: right application in tiles of 16 rows (see FusedTile)
+    FusedTile TL;
+    if (tile_ld > 0) {
+        if (M.D % 16 || tile_nrb < 1 || tile_ld < 16 * tile_nrb) return hipErrorInvalidValue;
+        TL.ld = tile_ld;
+        TL.nrb = tile_nrb;
+        TL.P = M.D / 16;
+    }
     if (ncols == 0) return hipSuccess;
     if (!fused_supported(dt, M)) return hipErrorInvalidValue;
     const int bz = (beta.x == 0.0 && beta.y == 0.0);
@@ -278,9 +306,9 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
     const int ncu = device_cu_count();
     const int64_t per_cu = std::max<int64_t>(1, (160 * 1024) / (int64_t)lds);
     const unsigned g = (unsigned)std::min<int64_t>(ncols, (int64_t)ncu * per_cu);
-#define QOB_FU3(RR, NTM, EP, GP)                                                                             \
+#define QOB_FU3(RR, NTM, EP, GP, TI)                                                                             \
     do {                                                                                                     \
-        auto kfn = k_lt_fused<RR, NTM, EP, GP>;                                                              \
+        auto kfn = k_lt_fused<RR, NTM, EP, GP, TI>;                                                          \
         static std::atomic<int> lds_set{0}; /* raise the limit once per kernel */                            \
         if (!lds_set.load()) {                                                                               \
             hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
@@ -289,14 +317,15 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
             lds_set.store(1);                                                                                \
         }                                                                                                    \
         hipLaunchKernelGGL(kfn, dim3(g), dim3(NTH), lds, s, M, (const cplx<RR> *)in, (cplx<RR> *)out, ncols, \
-                           alpha, beta, bz);                                                                 \
+                           TL, alpha, beta, bz);                                                             \
         return hipGetLastError();                                                                            \
     } while (0)
-#define QOB_FU(RR, NTM, EP)                                   \
-    do {                                                      \
-        if (g_fused_gpw == 1) QOB_FU3(RR, NTM, EP, 1);       \
-        else if (g_fused_gpw == 4) QOB_FU3(RR, NTM, EP, 4);  \
-        else QOB_FU3(RR, NTM, EP, 2);                        \
+#define QOB_FU(RR, NTM, EP)                                          \
+    do {                                                             \
+        if (tile_ld > 0) QOB_FU3(RR, NTM, EP, 2, true);              \
+        else if (g_fused_gpw == 1) QOB_FU3(RR, NTM, EP, 1, false);   \
+        else if (g_fused_gpw == 4) QOB_FU3(RR, NTM, EP, 4, false);   \
+        else QOB_FU3(RR, NTM, EP, 2, false);                         \
     } while (0)
     const int ntm = maxd / 16;
     if (dt == QOB_C64) {

@@ -254,7 +254,8 @@ hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncol
                                 const void *vals, double2 alpha, double2 beta, hipStream_t s);
 // out (D x ncols) = alpha * (A_n ... A_1) in + beta * out
 hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols,
-                           double2 alpha, double2 beta, hipStream_t s);
+                           double2 alpha, double2 beta, hipStream_t s, int64_t tile_ld = 0,
+                           int64_t tile_nrb = 1);
 
 // Fused reductions over a square gather plan (FP64 partials, one per block):
 //   mode 0: sum_J conj(u[J]) (H v)[J]          (expect / bilinear on a Ket)
