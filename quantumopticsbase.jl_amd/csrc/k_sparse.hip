@@ -161,6 +161,7 @@ __global__ void __launch_bounds__(256) k_sell_left_closed(int nslices, const int
     const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
     const uint32_t sg = lb % nsg, cg = lb / nsg;
     const int t = threadIdx.x;
+    if ((int64_t)sg * 4 >= nslices) return;  // workgroup-uniform, before any barrier
     const int slice = (int)(sg * 4 + (t >> 6));  // nslices is a multiple of 4: every lane has a slice
     const int r = t & 63;
     const int32_t row = rows[(int64_t)sg * 256 + t];
