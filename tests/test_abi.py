@@ -51,3 +51,20 @@ def test_fastdiv_exact(tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0 and out.stdout.startswith("OK"), out.stdout + out.stderr
+
+
+def _build_capi(tmp_path):
+    """Compile tests/capi_check.c as C11 against include/qob_hip.h and link
+    it to the built library (no Python involved)."""
+    from qobhip import _lib
+    libdir = Path(_lib.LIB_PATH).parent
+    exe = tmp_path / "capi_check"
+    r = subprocess.run(["gcc", "-std=c11", "-O1", "-Wall", "-Wextra", "-Werror", "-I", str(ROOT / "include"),
+                        str(ROOT / "tests" / "capi_check.c"), "-o", str(exe), "-L", str(libdir), "-lqob_hip",
+                        f"-Wl,-rpath,{libdir}", "-lm"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return exe
+
+
+def test_header_is_c_and_a_c_program_links(tmp_path):
+    assert _build_capi(tmp_path).exists()
