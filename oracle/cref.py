@@ -74,8 +74,11 @@ class LazySumKet:
         psi = np.ascontiguousarray(psi, dtype=np.complex128)
         assert result.dtype == np.complex128 and result.flags.c_contiguous and result.size == self.D
         a, b = complex(alpha), complex(beta)
-        lib().qobref_lazysum_ket(a.real, a.imag, len(self.shape), _p(self.shape), self.nterms, _p(self.coef),
-                                 _p(self.term_off), _p(self.fsite), _p(self.fdim), _p(self.cp_off), _p(self.nz_off),
-                                 _p(self.colptr), _p(self.rowval), _p(self.nzval), _p(psi), b.real, b.imag,
-                                 _p(result))
+        if getattr(self, "_args", None) is None:
+            # the operator's arrays never change: their pointers are built once,
+            # so a call spends its time in C (threads calling it run in parallel)
+            self._args = (len(self.shape), _p(self.shape), self.nterms, _p(self.coef), _p(self.term_off),
+                          _p(self.fsite), _p(self.fdim), _p(self.cp_off), _p(self.nz_off), _p(self.colptr),
+                          _p(self.rowval), _p(self.nzval))
+        lib().qobref_lazysum_ket(a.real, a.imag, *self._args, psi.ctypes.data, b.real, b.imag, result.ctypes.data)
         return result
