@@ -85,7 +85,23 @@ def probe(ctx, src, dst, step):
     return None
 
 
+def host_cpu() -> str:
+    """Model name and logical CPU count of the host the CPU baseline ran on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
 def base_result(args, value, unit, ms_per_step, dtype, data, config, roofline, cpu):
+    if cpu is not None:
+        cpu = {**cpu, "host": host_cpu()}
     return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": dtype, "data": data, "config": config, "roofline": roofline,
