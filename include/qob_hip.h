@@ -245,7 +245,10 @@ qob_status qob_graph_destroy(qob_graph g);
  * "sell_cj", "sell_jchunk", "sell_unr", "csc_rb", "csc_cc", "csc_remap",
  * "csc_rows", "csc_order", "qubit_tbits", "qubit_nth", "qubit_grid",
  * "qubit_remap", "qubit_split", "qubit_enable", "modek_nth", "fused_gpw",
- * "fused_lt", "nt_store", "copy_variant".  The project's
+ * "fused_lt", "fused_tiles", "fused_resident", "nt_store", "copy_variant",
+ * "sell_align", "sell_closed", "sell_cjchunk", "sell_cremap", "csc_groups",
+ * "csc_gkernel", "csc_gremap", "csc_waves", "csc_sweep", "ls_materialize".
+ * The project's
  * autotuner (qobhip/tune.py) sweeps them and persists a table; unknown names
  * return QOB_EARG.  Settings are process-wide. */
 qob_status qob_tune_set(const char *name, int64_t value);

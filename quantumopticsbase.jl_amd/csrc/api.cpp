@@ -34,7 +34,7 @@ namespace qob {
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
-extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles;
+extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles, g_fused_resident;
 extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel, g_csc_gremap, g_sell_cremap;
 
 // ------------------------------------------------------------------ errors
@@ -2568,7 +2568,7 @@ static int *tune_slot(const char *name) {
                {"sell_cjchunk", &g_sell_cjchunk}, {"csc_groups", &g_csc_groups},
                {"csc_gkernel", &g_csc_gkernel},   {"csc_gremap", &g_csc_gremap},
                {"sell_cremap", &g_sell_cremap},   {"ls_materialize", &g_ls_materialize},
-               {"fused_tiles", &g_fused_tiles}};
+               {"fused_tiles", &g_fused_tiles},   {"fused_resident", &g_fused_resident}};
     for (auto &e : tbl)
         if (name && strcmp(name, e.n) == 0) return e.p;
     return nullptr;
@@ -2597,7 +2597,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
-         !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles")) &&
+         !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles") ||
+         !strcmp(name, "fused_resident")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)

@@ -73,6 +73,23 @@ struct FusedTile {
     int P = 0;
 };
 
+// a[s] for a wave-uniform s through constant indices only: each a[i] is a
+// scalar kernel-argument load, where a[s] would become a vector load that
+// retires behind every global load issued before it.  The empty asm keeps
+// each a[i] opaque (in an SGPR), so the compiler cannot fold the selects back
+// into a[s].
+__device__ __forceinline__ int mode_sel(const int (&a)[FUSED_MAXMODES], int s) {
+    int r = a[0];
+    asm("" : "+s"(r));
+#pragma unroll
+    for (int i = 1; i < FUSED_MAXMODES; ++i) {
+        int v = a[i];
+        asm("" : "+s"(v));
+        r = (s == i) ? v : r;
+    }
+    return r;
+}
+
 template <bool TILE>
 __device__ __forceinline__ int64_t fused_addr(const FusedTile &T, int D, int64_t c, int e) {
     if (!TILE) return c * (int64_t)D + e;
@@ -80,7 +97,13 @@ __device__ __forceinline__ int64_t fused_addr(const FusedTile &T, int D, int64_t
     return rb * 16 + (e & 15) + T.ld * ((int64_t)(e >> 4) + (int64_t)T.P * q);
 }
 
-template <typename R, int MAXNTM, int EPT, int GPW, bool TILE>
+// RES = true: every factor stays in LDS for the whole launch (mode s at
+// offset sum_{t<s} d_t^2, ld = d), staged once per workgroup.  The mode loop
+// then issues no global loads besides the next column's prefetch, so that
+// prefetch stays in flight across all modes (loads retire in order: a
+// factor load issued after it would have to wait for the whole column), and
+// each mode needs one barrier instead of two.
+template <typename R, int MAXNTM, int EPT, int GPW, bool TILE, bool RES = false>
 __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *__restrict__ in,
                                                    cplx<R> *__restrict__ out, int64_t ncols, FusedTile TL,
                                                    double2 alpha_, double2 beta_, int beta_zero) {
@@ -91,7 +114,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
     const int D = M.D;
     const int Dp = D + (D >> 4);
     R *im = re + Dp;
-    T *Al = (T *)(im + Dp + (Dp & 1));  // factor of the current mode, ld = d + 1
+    T *Al = (T *)(im + Dp + (Dp & 1));  // factor of the current mode, ld = d + 1 (RES: all factors)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
@@ -100,7 +123,16 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
     // load latency hides behind the current mode's MFMAs
     constexpr int APRE = (16 * MAXNTM) * (16 * MAXNTM) / NTH > 0 ? (16 * MAXNTM) * (16 * MAXNTM) / NTH : 1;
     T apre[APRE];
-    {
+    if constexpr (RES) {
+        // ordered before every read of the factors by the first mode's barrier
+        int off = 0;
+        for (int s = 0; s < M.nmodes; ++s) {
+            const T *__restrict__ As = (const T *)M.A[s];
+            const int dd = M.d[s] * M.d[s];
+            for (int e = tid; e < dd; e += NTH) Al[off + e] = As[e];
+            off += dd;
+        }
+    } else {
         const T *__restrict__ A0 = (const T *)M.A[0];
         const int dd = M.d[0] * M.d[0];
 #pragma unroll
@@ -116,14 +148,22 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
-            if (e < D) pre[k] = in[fused_addr<TILE>(TL, D, col, e)];
+            if constexpr (RES) pre[k] = in[fused_addr<TILE>(TL, D, col, e < D ? e : D - 1)];
+            else if (e < D) pre[k] = in[fused_addr<TILE>(TL, D, col, e)];
         }
     }
     for (; col < ncols; col += gridDim.x) {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
-            if (e < D) {
+            if constexpr (RES) {
+                // branch-free, so the wait for pre[k] counts exactly and does
+                // not also wait for the previous column's output stores; lanes
+                // past D write slot 16, a padding hole no element maps to
+                const int idx = e < D ? pad(e) : 16;
+                re[idx] = pre[k].x;
+                im[idx] = pre[k].y;
+            } else if (e < D) {
                 re[pad(e)] = pre[k].x;
                 im[pad(e)] = pre[k].y;
             }
@@ -134,14 +174,22 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 #pragma unroll
             for (int k = 0; k < EPT; ++k) {
                 const int e = tid + k * NTH;
-                if (e < D) pre[k] = in[fused_addr<TILE>(TL, D, nxt, e)];
+                if constexpr (RES) {
+                    // unconditional (clamped) loads: no branch between them and
+                    // their use, so the wait before the use counts exactly
+                    pre[k] = in[fused_addr<TILE>(TL, D, nxt, e < D ? e : D - 1)];
+                } else if (e < D) {
+                    pre[k] = in[fused_addr<TILE>(TL, D, nxt, e)];
+                }
             }
         }
+        int aoff = 0;  // RES: offset of the current mode's factor
 
         for (int s = 0; s < M.nmodes; ++s) {
-            const int n1 = M.n1[s], d = M.d[s];
+            const int n1 = RES ? mode_sel(M.n1, s) : M.n1[s], d = RES ? mode_sel(M.d, s) : M.d[s];
             const int lda = d + 1;
             __syncthreads();  // previous mode written back, column loaded
+            if constexpr (!RES) {
 #pragma unroll
             for (int k = 0; k < APRE; ++k) {
                 const int e = tid + k * NTH;
@@ -163,6 +211,10 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
                 }
             }
             __syncthreads();
+            }
+            const T *Am = RES ? Al + aoff : Al;
+            const int ldam = RES ? d : lda;
+            aoff += d * d;
             const int ntm = d >> 4, ngroups = (D / d) >> 4;
             const int nks = d >> 2;
             for (int g0 = wave; g0 < ngroups; g0 += NW * GPW) {
@@ -193,7 +245,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 #pragma unroll
                     for (int mt = 0; mt < MAXNTM; ++mt) {
                         if (mt < ntm) {
-                            const T a = Al[(mt * 16 + lr) + (4 * ks + lk) * lda];
+                            const T a = Am[(mt * 16 + lr) + (4 * ks + lk) * ldam];
 #pragma unroll
                             for (int q = 0; q < GPW; ++q) {
                                 accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
@@ -256,13 +308,23 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 }  // namespace
 
 int g_fused_gpw = 2;  // fibre groups in flight per wave: 1, 2 or 4
-int g_fused_tiles = 1;  // 1: right applications run the fused kernel on 16-row tiles
+int g_fused_tiles = 1;
+int g_fused_resident = 1;  // 1: factors resident in LDS when planes + factors fit (two fibre groups per wave)  // 1: right applications run the fused kernel on 16-row tiles
 
 // LDS bytes for a column of D elements (padded planes)
 static size_t fused_lds(qob_dtype dt, int D, int dmax) {
     const size_t r = (dt == QOB_C64 ? 8 : 4);
     const size_t dp = (size_t)(D + (D >> 4));
     return (2 * dp + (dp & 1)) * r + (size_t)dmax * (dmax + 1) * 2 * r;
+}
+
+// resident factors: planes plus every factor at ld = d
+static size_t fused_lds_res(qob_dtype dt, const FusedModes &M) {
+    const size_t r = (dt == QOB_C64 ? 8 : 4);
+    const size_t dp = (size_t)(M.D + (M.D >> 4));
+    size_t a = 0;
+    for (int s = 0; s < M.nmodes; ++s) a += (size_t)M.d[s] * M.d[s];
+    return (2 * dp + (dp & 1)) * r + a * 2 * r;
 }
 
 static int fused_dmax(const FusedModes &M) {
@@ -301,7 +363,8 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
     if (!fused_supported(dt, M)) return hipErrorInvalidValue;
     const int bz = (beta.x == 0.0 && beta.y == 0.0);
     const int maxd = fused_dmax(M);
-    const size_t lds = fused_lds(dt, M.D, maxd);
+    const bool res = g_fused_resident && (tile_ld > 0 || g_fused_gpw == 2) && fused_lds_res(dt, M) <= 160 * 1024;
+    const size_t lds = res ? fused_lds_res(dt, M) : fused_lds(dt, M.D, maxd);
     const int ept = (M.D + NTH - 1) / NTH;
     const int ncu = device_cu_count();
     const int64_t per_cu = std::max<int64_t>(1, (160 * 1024) / (int64_t)lds);
@@ -320,8 +383,26 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
                            TL, alpha, beta, bz);                                                             \
         return hipGetLastError();                                                                            \
     } while (0)
+#define QOB_FU3R(RR, NTM, EP, GP, TI)                                                                             \
+    do {                                                                                                     \
+        auto kfn = k_lt_fused<RR, NTM, EP, GP, TI, true>;                                                          \
+        static std::atomic<int> lds_set{0}; /* raise the limit once per kernel */                            \
+        if (!lds_set.load()) {                                                                               \
+            hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                160 * 1024);                                                 \
+            if (e_ != hipSuccess) return e_;                                                                 \
+            lds_set.store(1);                                                                                \
+        }                                                                                                    \
+        hipLaunchKernelGGL(kfn, dim3(g), dim3(NTH), lds, s, M, (const cplx<RR> *)in, (cplx<RR> *)out, ncols, \
+                           TL, alpha, beta, bz);                                                             \
+        return hipGetLastError();                                                                            \
+    } while (0)
 #define QOB_FU(RR, NTM, EP)                                          \
     do {                                                             \
+        if (res) {                                                   \
+            if (tile_ld > 0) QOB_FU3R(RR, NTM, EP, 2, true);         \
+            QOB_FU3R(RR, NTM, EP, 2, false);                         \
+        }                                                            \
         if (tile_ld > 0) QOB_FU3(RR, NTM, EP, 2, true);              \
         else if (g_fused_gpw == 1) QOB_FU3(RR, NTM, EP, 1, false);   \
         else if (g_fused_gpw == 4) QOB_FU3(RR, NTM, EP, 4, false);   \
@@ -339,6 +420,7 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
     }
     return hipErrorInvalidValue;
 #undef QOB_FU
+#undef QOB_FU3R
 #undef QOB_FU3
     return hipGetLastError();
 }

@@ -419,6 +419,43 @@ def test_fused_equals_chain(dtype):
     assert rel(outs[0], outs[1]) < tol
 
 
+@pytest.mark.parametrize("shape,idx", [((16, 16), (1, 2)),            # D = 256: lanes past D in the column
+                                       ((16, 3, 16), (1, 3)),         # D = 768: second element per lane partly past D
+                                       ((16, 32, 16), (1, 2, 3)),     # c64: planes + factors fill the 160 KiB LDS
+                                       ((64, 16), (1, 2))])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_fused_resident_factors(shape, idx, dtype, tol):
+    """Factors resident in LDS for the whole launch against the per-mode
+    staging: bitwise equal (same products in the same order), left on
+    columns and right on 16-row tiles, and against the oracle."""
+    g = np.random.default_rng(90 + sum(shape))
+    lt = rand_lt(g, shape, shape, idx, ("dense",) * len(idx))
+    D = int(np.prod(shape))
+    B = cg(g, D, 5)
+    Bt = (g.standard_normal((64, D)) + 1j * g.standard_normal((64, D))) / np.sqrt(D)
+    R0 = g.standard_normal((D, 5)) + 0j
+    want = R0.copy()
+    O.mul_op_left(want, lt, B, 0.6 + 0.2j, -0.5j)
+    want_r = np.zeros((64, D), complex)
+    O.mul_op_right(want_r, Bt, lt, 1.0, 0.0)
+    got = {}
+    try:
+        for res in (1, 0):
+            _tune("fused_resident", res)
+            q = to_q(lt)
+            R = Q.Operator(q.basis_l, gb(5), R0, dtype)
+            Q.mul_(R, q, Q.Operator(q.basis_r, gb(5), B, dtype), 0.6 + 0.2j, -0.5j)
+            Rt = Q.Operator(gb(64), q.basis_r, np.full((64, D), np.nan, complex), dtype)
+            Q.mul_(Rt, Q.Operator(gb(64), q.basis_l, Bt, dtype), q, 1.0, 0.0)
+            got[res] = (R.to_numpy(), Rt.to_numpy())
+    finally:
+        _tune("fused_resident", 1)
+    assert rel(got[1][0], want) < tol
+    assert rel(got[1][1], want_r) < tol
+    assert np.array_equal(got[1][0], got[0][0])
+    assert np.array_equal(got[1][1], got[0][1])
+
+
 @pytest.mark.parametrize("rows_tunable,waves,sweep", [(16, 1, 0), (32, 1, 0), (64, 1, 0), (64, 1, 1), (64, 2, 0),
                                                       (64, 4, 1), (64, 4, 2)])
 @pytest.mark.parametrize("cc", [1, 3])

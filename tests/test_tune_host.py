@@ -60,6 +60,22 @@ def test_invalid_tunable_values_are_rejected():
             T.set_(name, bad)
 
 
+def test_every_documented_and_searched_tunable_exists():
+    """Each name the header lists and each name the tuner searches is known
+    to the library: get returns a value and setting it back is accepted."""
+    import re
+    from pathlib import Path
+    text = Path(__file__).resolve().parents[1].joinpath("include", "qob_hip.h").read_text()
+    block = text[text.index("Launch-shape knobs"):text.index("The project's")]
+    names = set(re.findall(r'"([a-z0-9_]+)"', block))
+    names |= {n for space in T.SPACES.values() for n in space}
+    assert {"fused_resident", "fused_tiles", "sell_closed", "csc_groups"} <= names
+    for n in sorted(names):
+        v = T.get(n)
+        T.set_(n, v)
+        assert T.get(n) == v
+
+
 def test_min_gain_keeps_start_against_noise():
     space = {"a": [1, 2, 3]}
     costs = {1: 1.0, 2: 0.99, 3: 1.2}  # 1% better only
