@@ -276,7 +276,10 @@ def run_sparse6400(args):
     # the Hamiltonian's pattern splits into components of at most 4 rows, so
     # with closed blocks enabled the left product runs k_sell_left_closed
     left_kernel = "k_sell_left_closed" if T.get("sell_closed") else "k_sell_left_rr"
-    roof = {"bound": "hbm", "kernel": left_kernel if dom == "left" else "k_csc_right",
+    # and into components of at most 4 columns: the right product runs the
+    # component-group kernel when groups are built and enabled
+    right_kernel = "k_csc_right_groups" if T.get("csc_groups") and T.get("csc_gkernel") else "k_csc_right"
+    roof = {"bound": "hbm", "kernel": left_kernel if dom == "left" else right_kernel,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_of(f"csr_{dom}"), "algorithmic_bytes_per_launch": bytes_per_launch}
     return base_result(args, applies_per_s, "applies/s", ms_per_step, "f64",
