@@ -97,6 +97,107 @@ __device__ __forceinline__ int64_t fused_addr(const FusedTile &T, int D, int64_t
     return rb * 16 + (e & 15) + T.ld * ((int64_t)(e >> 4) + (int64_t)T.P * q);
 }
 
+// One mode of one column: each wave takes groups of 16 fibres (GPW at a
+// time), forms every 16-row output tile of the group on MFMA and writes it
+// back into the planes, or (last mode) to the output column.  DM > 0 fixes
+// the mode size d at compile time; DM = 0 reads it at run time.
+template <typename R, int MAXNTM, int GPW, bool TILE, int DM>
+__device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int ldam, int n1, int d_rt, int D,
+                                           bool last, cplx<R> *__restrict__ out, int64_t col, const FusedTile &TL,
+                                           cplx<R> alpha, cplx<R> beta, int beta_zero) {
+    using T = cplx<R>;
+    using V = typename mf<R>::v4;
+    constexpr int NTM = DM > 0 ? DM / 16 : MAXNTM;  // accumulator tiles per group
+    constexpr int KS = DM > 0 ? DM / 4 : MAXNTM * 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int d = DM > 0 ? DM : d_rt;
+    const int ntm = d >> 4, ngroups = (D / d) >> 4;
+    const int nks = d >> 2;
+    for (int g0 = wave; g0 < ngroups; g0 += NW * GPW) {
+        V accr[GPW][NTM], acci[GPW][NTM];
+        int base[GPW];
+#pragma unroll
+        for (int q = 0; q < GPW; ++q) {
+            const int g = g0 + q * NW;
+            const int f = g * 16 + lr;
+            const int i1 = f % n1, i3 = f / n1;
+            base[q] = i1 + n1 * d * i3;
+#pragma unroll
+            for (int mt = 0; mt < NTM; ++mt) {
+                accr[q][mt] = V{0, 0, 0, 0};
+                acci[q][mt] = V{0, 0, 0, 0};
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            if (ks >= nks) break;
+            R xr[GPW], xi[GPW];
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) {
+                const int idx = (g0 + q * NW < ngroups) ? pad(base[q] + n1 * (4 * ks + lk)) : 0;
+                xr[q] = re[idx];
+                xi[q] = im[idx];
+            }
+#pragma unroll
+            for (int mt = 0; mt < NTM; ++mt) {
+                if (mt < ntm) {
+                    const T a = Am[(mt * 16 + lr) + (4 * ks + lk) * ldam];
+#pragma unroll
+                    for (int q = 0; q < GPW; ++q) {
+                        accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
+                        accr[q][mt] = mf<R>::mma(a.y, -xi[q], accr[q][mt]);
+                        acci[q][mt] = mf<R>::mma(a.x, xi[q], acci[q][mt]);
+                        acci[q][mt] = mf<R>::mma(a.y, xr[q], acci[q][mt]);
+                    }
+                }
+            }
+        }
+        if (!last) {
+            // the group's inputs are no longer needed by anyone: write back
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) {
+                if (g0 + q * NW < ngroups) {
+#pragma unroll
+                    for (int mt = 0; mt < NTM; ++mt) {
+                        if (mt < ntm) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int a = mt * 16 + mf<R>::row(lane, r);
+                                const int idx = pad(base[q] + n1 * a);
+                                re[idx] = accr[q][mt][r];
+                                im[idx] = acci[q][mt][r];
+                            }
+                        }
+                    }
+                }
+            }
+        } else {
+            // last mode: the tiles go from the accumulators straight to the
+            // output column (out = alpha y + beta out); the stores drain
+            // while the next column is being computed
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) {
+                if (g0 + q * NW < ngroups) {
+#pragma unroll
+                    for (int mt = 0; mt < NTM; ++mt) {
+                        if (mt < ntm) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int e = base[q] + n1 * (mt * 16 + mf<R>::row(lane, r));
+                                T v = cmul(alpha, mk((R)accr[q][mt][r], (R)acci[q][mt][r]));
+                                const int64_t ge = fused_addr<TILE>(TL, D, col, e);
+                                if (!beta_zero) v = cfma(beta, out[ge], v);
+                                st_stream(out + ge, v);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // RES = true: every factor stays in LDS for the whole launch (mode s at
 // offset sum_{t<s} d_t^2, ld = d), staged once per workgroup.  The mode loop
 // then issues no global loads besides the next column's prefetch, so that
@@ -215,90 +316,21 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
             const T *Am = RES ? Al + aoff : Al;
             const int ldam = RES ? d : lda;
             aoff += d * d;
-            const int ntm = d >> 4, ngroups = (D / d) >> 4;
-            const int nks = d >> 2;
-            for (int g0 = wave; g0 < ngroups; g0 += NW * GPW) {
-                V accr[GPW][MAXNTM], acci[GPW][MAXNTM];
-                int base[GPW];
-#pragma unroll
-                for (int q = 0; q < GPW; ++q) {
-                    const int g = g0 + q * NW;
-                    const int f = g * 16 + lr;
-                    const int i1 = f % n1, i3 = f / n1;
-                    base[q] = i1 + n1 * d * i3;
-#pragma unroll
-                    for (int mt = 0; mt < MAXNTM; ++mt) {
-                        accr[q][mt] = V{0, 0, 0, 0};
-                        acci[q][mt] = V{0, 0, 0, 0};
-                    }
-                }
-#pragma unroll
-                for (int ks = 0; ks < MAXNTM * 4; ++ks) {
-                    if (ks >= nks) break;
-                    R xr[GPW], xi[GPW];
-#pragma unroll
-                    for (int q = 0; q < GPW; ++q) {
-                        const int idx = (g0 + q * NW < ngroups) ? pad(base[q] + n1 * (4 * ks + lk)) : 0;
-                        xr[q] = re[idx];
-                        xi[q] = im[idx];
-                    }
-#pragma unroll
-                    for (int mt = 0; mt < MAXNTM; ++mt) {
-                        if (mt < ntm) {
-                            const T a = Am[(mt * 16 + lr) + (4 * ks + lk) * ldam];
-#pragma unroll
-                            for (int q = 0; q < GPW; ++q) {
-                                accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
-                                accr[q][mt] = mf<R>::mma(a.y, -xi[q], accr[q][mt]);
-                                acci[q][mt] = mf<R>::mma(a.x, xi[q], acci[q][mt]);
-                                acci[q][mt] = mf<R>::mma(a.y, xr[q], acci[q][mt]);
-                            }
-                        }
-                    }
-                }
-                if (s + 1 < M.nmodes) {
-                    // the group's inputs are no longer needed by anyone: write back
-#pragma unroll
-                    for (int q = 0; q < GPW; ++q) {
-                        if (g0 + q * NW < ngroups) {
-#pragma unroll
-                            for (int mt = 0; mt < MAXNTM; ++mt) {
-                                if (mt < ntm) {
-#pragma unroll
-                                    for (int r = 0; r < 4; ++r) {
-                                        const int a = mt * 16 + mf<R>::row(lane, r);
-                                        const int idx = pad(base[q] + n1 * a);
-                                        re[idx] = accr[q][mt][r];
-                                        im[idx] = acci[q][mt][r];
-                                    }
-                                }
-                            }
-                        }
-                    }
+            const bool last = s + 1 == M.nmodes;
+            // mode size as a template argument: the k-step and tile loops
+            // unroll to a fixed MFMA sequence with no branches in between
+            // (factors up to 32 wide).  Kept at run-time sizes: kernels with
+            // 64-wide factors and the ComplexF64 tile variant, which spill
+            // further with fixed sizes (the tile variant measured 1.30
+            // against 1.18 ms).
+            if constexpr (RES && MAXNTM <= 2 && !(TILE && sizeof(R) == 8)) {
+                if (d == 16) {
+                    fused_mode<R, MAXNTM, GPW, TILE, 16>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 } else {
-                    // last mode: the tiles go from the accumulators straight
-                    // to the output column (out = alpha y + beta out); the
-                    // stores drain while the next column is being computed
-                    T *dst = out;
-#pragma unroll
-                    for (int q = 0; q < GPW; ++q) {
-                        if (g0 + q * NW < ngroups) {
-#pragma unroll
-                            for (int mt = 0; mt < MAXNTM; ++mt) {
-                                if (mt < ntm) {
-#pragma unroll
-                                    for (int r = 0; r < 4; ++r) {
-                                        const int e = base[q] + n1 * (mt * 16 + mf<R>::row(lane, r));
-                                        T v = cmul(alpha, mk((R)accr[q][mt][r], (R)acci[q][mt][r]));
-                                        const int64_t ge = fused_addr<TILE>(TL, D, col, e);
-                                        if (!beta_zero) v = cfma(beta, dst[ge], v);
-                                        st_stream(dst + ge, v);
-                                    }
-                                }
-                            }
-                        }
-                    }
+                    fused_mode<R, MAXNTM, GPW, TILE, 32>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 }
+            } else {
+                fused_mode<R, MAXNTM, GPW, TILE, 0>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
             }
         }
         __syncthreads();  // every wave is done reading this column's planes
@@ -308,8 +340,8 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 }  // namespace
 
 int g_fused_gpw = 2;  // fibre groups in flight per wave: 1, 2 or 4
-int g_fused_tiles = 1;
-int g_fused_resident = 1;  // 1: factors resident in LDS when planes + factors fit (two fibre groups per wave)  // 1: right applications run the fused kernel on 16-row tiles
+int g_fused_tiles = 1;     // 1: right applications run the fused kernel on 16-row tiles
+int g_fused_resident = 1;  // 1: factors resident in LDS when planes + factors fit (two fibre groups per wave)
 
 // LDS bytes for a column of D elements (padded planes)
 static size_t fused_lds(qob_dtype dt, int D, int dmax) {
