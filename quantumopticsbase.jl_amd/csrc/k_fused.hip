@@ -11,9 +11,13 @@
 //      reads for every mode stride);
 //   2. for each factor A (d x d, d a multiple of 16) the mode product
 //      Y = A x_k X is a GEMM  (d x d) * (d x D/d): 16 x 16 output tiles on
-//      v_mfma_{f64,f32}_16x16x4, four real MFMAs per complex k-step
-//      (re*re - im*im, re*im + im*re), accumulators in registers; all waves
-//      finish reading before any tile is written back into the planes;
+//      v_mfma_{f64,f32}_16x16x4, accumulators in registers; all waves
+//      finish reading before any tile is written back into the planes.
+//      Four real MFMAs per complex k-step (re*re - im*im, re*im + im*re);
+//      ComplexF32 kernels with room for a third accumulator use three
+//      (Gauss's product: P1 = Ar Xr, P2 = Ai Xi, P3 = (Ar + Ai)(Xr + Xi);
+//      Re = P1 - P2, Im = P3 - P1 - P2); the ComplexF64 kernels have no
+//      registers left for it;
 //   3. the column is written once: out = alpha * Y + beta * out.
 // ComplexF64 uses FP64 MFMA; ComplexF32 uses FP32 MFMA with FP32
 // accumulation.  Columns are distributed over persistent workgroups.
@@ -100,8 +104,9 @@ __device__ __forceinline__ int64_t fused_addr(const FusedTile &T, int D, int64_t
 // One mode of one column: each wave takes groups of 16 fibres (GPW at a
 // time), forms every 16-row output tile of the group on MFMA and writes it
 // back into the planes, or (last mode) to the output column.  DM > 0 fixes
-// the mode size d at compile time; DM = 0 reads it at run time.
-template <typename R, int MAXNTM, int GPW, bool TILE, int DM>
+// the mode size d at compile time; DM = 0 reads it at run time.  M3: three
+// real products per complex k-step (see the file header).
+template <typename R, int MAXNTM, int GPW, bool TILE, int DM, bool M3>
 __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int ldam, int n1, int d_rt, int D,
                                            bool last, cplx<R> *__restrict__ out, int64_t col, const FusedTile &TL,
                                            cplx<R> alpha, cplx<R> beta, int beta_zero) {
@@ -115,7 +120,7 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
     const int ntm = d >> 4, ngroups = (D / d) >> 4;
     const int nks = d >> 2;
     for (int g0 = wave; g0 < ngroups; g0 += NW * GPW) {
-        V accr[GPW][NTM], acci[GPW][NTM];
+        V accr[GPW][NTM], acci[GPW][NTM], accs[GPW][NTM];  // accs: P3 (M3 only)
         int base[GPW];
 #pragma unroll
         for (int q = 0; q < GPW; ++q) {
@@ -127,6 +132,7 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
             for (int mt = 0; mt < NTM; ++mt) {
                 accr[q][mt] = V{0, 0, 0, 0};
                 acci[q][mt] = V{0, 0, 0, 0};
+                accs[q][mt] = V{0, 0, 0, 0};
             }
         }
 #pragma unroll
@@ -145,11 +151,29 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
                     const T a = Am[(mt * 16 + lr) + (4 * ks + lk) * ldam];
 #pragma unroll
                     for (int q = 0; q < GPW; ++q) {
-                        accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
-                        accr[q][mt] = mf<R>::mma(a.y, -xi[q], accr[q][mt]);
-                        acci[q][mt] = mf<R>::mma(a.x, xi[q], acci[q][mt]);
-                        acci[q][mt] = mf<R>::mma(a.y, xr[q], acci[q][mt]);
+                        if constexpr (M3) {
+                            accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
+                            acci[q][mt] = mf<R>::mma(a.y, xi[q], acci[q][mt]);
+                            accs[q][mt] = mf<R>::mma(a.x + a.y, xr[q] + xi[q], accs[q][mt]);
+                        } else {
+                            accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
+                            accr[q][mt] = mf<R>::mma(a.y, -xi[q], accr[q][mt]);
+                            acci[q][mt] = mf<R>::mma(a.x, xi[q], acci[q][mt]);
+                            acci[q][mt] = mf<R>::mma(a.y, xr[q], acci[q][mt]);
+                        }
                     }
+                }
+            }
+        }
+        if constexpr (M3) {
+            // (P1, P2, P3) -> (Re, Im)
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) {
+#pragma unroll
+                for (int mt = 0; mt < NTM; ++mt) {
+                    const V p1 = accr[q][mt], p2 = acci[q][mt];
+                    accr[q][mt] = p1 - p2;
+                    acci[q][mt] = accs[q][mt] - p1 - p2;
                 }
             }
         }
@@ -286,6 +310,10 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
         }
         int aoff = 0;  // RES: offset of the current mode's factor
 
+        // Gauss's product in FP32 where its third accumulator adds no
+        // register spills (factors up to 32 wide, up to 16 elements per
+        // lane, up to two fibre groups per wave)
+        constexpr bool M3 = sizeof(R) == 4 && MAXNTM <= 2 && EPT <= 16 && GPW <= 2;
         for (int s = 0; s < M.nmodes; ++s) {
             const int n1 = RES ? mode_sel(M.n1, s) : M.n1[s], d = RES ? mode_sel(M.d, s) : M.d[s];
             const int lda = d + 1;
@@ -325,12 +353,12 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
             // against 1.18 ms).
             if constexpr (RES && MAXNTM <= 2 && !(TILE && sizeof(R) == 8)) {
                 if (d == 16) {
-                    fused_mode<R, MAXNTM, GPW, TILE, 16>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                    fused_mode<R, MAXNTM, GPW, TILE, 16, M3>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 } else {
-                    fused_mode<R, MAXNTM, GPW, TILE, 32>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                    fused_mode<R, MAXNTM, GPW, TILE, 32, M3>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 }
             } else {
-                fused_mode<R, MAXNTM, GPW, TILE, 0>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                fused_mode<R, MAXNTM, GPW, TILE, 0, M3>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
             }
         }
         __syncthreads();  // every wave is done reading this column's planes
