@@ -2592,8 +2592,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
         return fail(QOB_EARG, "csc_waves must be 1, 2 or 4");
     if (!strcmp(name, "csc_rows") && value != 16 && value != 32 && value != 64)
         return fail(QOB_EARG, "csc_rows must be 16, 32 or 64");
-    if (!strcmp(name, "fused_gpw") && value != 1 && value != 2 && value != 4)
-        return fail(QOB_EARG, "fused_gpw must be 1, 2 or 4");
+    if (!strcmp(name, "fused_gpw") && value != 0 && value != 1 && value != 2 && value != 4)
+        return fail(QOB_EARG, "fused_gpw must be 0 (automatic), 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||

@@ -13,11 +13,11 @@
 //      Y = A x_k X is a GEMM  (d x d) * (d x D/d): 16 x 16 output tiles on
 //      v_mfma_{f64,f32}_16x16x4, accumulators in registers; all waves
 //      finish reading before any tile is written back into the planes.
-//      Four real MFMAs per complex k-step (re*re - im*im, re*im + im*re);
-//      ComplexF32 kernels with room for a third accumulator use three
-//      (Gauss's product: P1 = Ar Xr, P2 = Ai Xi, P3 = (Ar + Ai)(Xr + Xi);
-//      Re = P1 - P2, Im = P3 - P1 - P2); the ComplexF64 kernels have no
-//      registers left for it;
+//      Four real MFMAs per complex k-step (re*re - im*im, re*im + im*re),
+//      or three where the kernel has room for a third accumulator (Gauss's
+//      product: P1 = Ar Xr, P2 = Ai Xi, P3 = (Ar + Ai)(Xr + Xi);
+//      Re = P1 - P2, Im = P3 - P1 - P2): ComplexF32 with two fibre groups
+//      per wave, ComplexF64 with one;
 //   3. the column is written once: out = alpha * Y + beta * out.
 // ComplexF64 uses FP64 MFMA; ComplexF32 uses FP32 MFMA with FP32
 // accumulation.  Columns are distributed over persistent workgroups.
@@ -310,10 +310,12 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
         }
         int aoff = 0;  // RES: offset of the current mode's factor
 
-        // Gauss's product in FP32 where its third accumulator adds no
-        // register spills (factors up to 32 wide, up to 16 elements per
-        // lane, up to two fibre groups per wave)
-        constexpr bool M3 = sizeof(R) == 4 && MAXNTM <= 2 && EPT <= 16 && GPW <= 2;
+        // Gauss's product where its third accumulator adds no register
+        // spills: FP32 with factors up to 32 wide, up to 16 elements per
+        // lane and up to two fibre groups per wave; FP64 with resident
+        // factors up to 32 wide and one fibre group per wave
+        constexpr bool M3 = (sizeof(R) == 4 && MAXNTM <= 2 && EPT <= 16 && GPW <= 2) ||
+                            (sizeof(R) == 8 && MAXNTM <= 2 && GPW == 1 && RES);
         for (int s = 0; s < M.nmodes; ++s) {
             const int n1 = RES ? mode_sel(M.n1, s) : M.n1[s], d = RES ? mode_sel(M.d, s) : M.d[s];
             const int lda = d + 1;
@@ -351,7 +353,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
             // 64-wide factors and the ComplexF64 tile variant, which spill
             // further with fixed sizes (the tile variant measured 1.30
             // against 1.18 ms).
-            if constexpr (RES && MAXNTM <= 2 && !(TILE && sizeof(R) == 8)) {
+            if constexpr (RES && MAXNTM <= 2 && !(TILE && sizeof(R) == 8 && GPW == 2)) {
                 if (d == 16) {
                     fused_mode<R, MAXNTM, GPW, TILE, 16, M3>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 } else {
@@ -367,7 +369,9 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 
 }  // namespace
 
-int g_fused_gpw = 2;  // fibre groups in flight per wave: 1, 2 or 4
+// fibre groups in flight per wave: 1, 2 or 4; 0 (automatic): one for
+// ComplexF64 with resident factors up to 32 wide (Gauss's product), else two
+int g_fused_gpw = 0;
 int g_fused_tiles = 1;     // 1: right applications run the fused kernel on 16-row tiles
 int g_fused_resident = 1;  // 1: factors resident in LDS when planes + factors fit (two fibre groups per wave)
 
@@ -423,7 +427,9 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
     if (!fused_supported(dt, M)) return hipErrorInvalidValue;
     const int bz = (beta.x == 0.0 && beta.y == 0.0);
     const int maxd = fused_dmax(M);
-    const bool res = g_fused_resident && (tile_ld > 0 || g_fused_gpw == 2) && fused_lds_res(dt, M) <= 160 * 1024;
+    const bool fits_res = g_fused_resident && fused_lds_res(dt, M) <= 160 * 1024;
+    const int gpw = g_fused_gpw ? g_fused_gpw : (dt == QOB_C64 && maxd <= 32 && fits_res ? 1 : 2);
+    const bool res = fits_res && gpw <= 2;
     const size_t lds = res ? fused_lds_res(dt, M) : fused_lds(dt, M.D, maxd);
     const int ept = (M.D + NTH - 1) / NTH;
     const int ncu = device_cu_count();
@@ -460,12 +466,16 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
 #define QOB_FU(RR, NTM, EP)                                          \
     do {                                                             \
         if (res) {                                                   \
+            if (gpw == 1) {                                          \
+                if (tile_ld > 0) QOB_FU3R(RR, NTM, EP, 1, true);     \
+                QOB_FU3R(RR, NTM, EP, 1, false);                     \
+            }                                                        \
             if (tile_ld > 0) QOB_FU3R(RR, NTM, EP, 2, true);         \
             QOB_FU3R(RR, NTM, EP, 2, false);                         \
         }                                                            \
         if (tile_ld > 0) QOB_FU3(RR, NTM, EP, 2, true);              \
-        else if (g_fused_gpw == 1) QOB_FU3(RR, NTM, EP, 1, false);   \
-        else if (g_fused_gpw == 4) QOB_FU3(RR, NTM, EP, 4, false);   \
+        else if (gpw == 1) QOB_FU3(RR, NTM, EP, 1, false);           \
+        else if (gpw == 4) QOB_FU3(RR, NTM, EP, 4, false);           \
         else QOB_FU3(RR, NTM, EP, 2, false);                         \
     } while (0)
     const int ntm = maxd / 16;

@@ -4,7 +4,8 @@ No reference counterpart.  The kernels' launch shapes are runtime tunables
 of libqob_hip.so (qob_tune_set / qob_tune_get):
 
   mode-k products   modek_nth (fibres per workgroup), fused_gpw (fibre
-                    groups in flight per wave in the fused MFMA kernel),
+                    groups in flight per wave in the fused MFMA kernel;
+                    0 = by precision and factor width),
                     fused_resident (factors kept in LDS for the launch)
   qubit sums        qubit_tbits (tile = 2^tbits states), qubit_nth,
                     qubit_grid (persistent workgroups; 0 = 3 per CU for ComplexF64, 4 for ComplexF32),
@@ -42,7 +43,7 @@ from ._lib import call
 
 SPACES: Dict[str, Dict[str, List[int]]] = {
     "modek": {"modek_nth": [64, 128, 256, 512]},
-    "fused": {"fused_gpw": [1, 2, 4], "fused_resident": [1, 0]},
+    "fused": {"fused_gpw": [0, 1, 2, 4], "fused_resident": [1, 0]},
     "qubit": {"qubit_tbits": [10, 11, 12, 13], "qubit_nth": [256, 512, 1024], "qubit_grid": [0, 768, 1024, 2048],
               "qubit_remap": [0, 1], "qubit_split": [0, 1]},
     "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16, 32],

@@ -426,8 +426,10 @@ def test_fused_equals_chain(dtype):
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
 def test_fused_resident_factors(shape, idx, dtype, tol):
     """Factors resident in LDS for the whole launch against the per-mode
-    staging: bitwise equal (same products in the same order), left on
-    columns and right on 16-row tiles, and against the oracle."""
+    staging, with two fibre groups per wave in both: bitwise equal (same
+    products in the same order), left on columns and right on 16-row tiles.
+    The automatic launch shape (ComplexF64: one fibre group per wave and
+    Gauss's three-product form when resident) against the oracle."""
     g = np.random.default_rng(90 + sum(shape))
     lt = rand_lt(g, shape, shape, idx, ("dense",) * len(idx))
     D = int(np.prod(shape))
@@ -440,20 +442,23 @@ def test_fused_resident_factors(shape, idx, dtype, tol):
     O.mul_op_right(want_r, Bt, lt, 1.0, 0.0)
     got = {}
     try:
-        for res in (1, 0):
+        for gpw, res in ((2, 1), (2, 0), (0, 1)):
+            _tune("fused_gpw", gpw)
             _tune("fused_resident", res)
             q = to_q(lt)
             R = Q.Operator(q.basis_l, gb(5), R0, dtype)
             Q.mul_(R, q, Q.Operator(q.basis_r, gb(5), B, dtype), 0.6 + 0.2j, -0.5j)
             Rt = Q.Operator(gb(64), q.basis_r, np.full((64, D), np.nan, complex), dtype)
             Q.mul_(Rt, Q.Operator(gb(64), q.basis_l, Bt, dtype), q, 1.0, 0.0)
-            got[res] = (R.to_numpy(), Rt.to_numpy())
+            got[gpw, res] = (R.to_numpy(), Rt.to_numpy())
     finally:
         _tune("fused_resident", 1)
-    assert rel(got[1][0], want) < tol
-    assert rel(got[1][1], want_r) < tol
-    assert np.array_equal(got[1][0], got[0][0])
-    assert np.array_equal(got[1][1], got[0][1])
+        _tune("fused_gpw", 0)
+    for key in ((2, 1), (0, 1)):
+        assert rel(got[key][0], want) < tol
+        assert rel(got[key][1], want_r) < tol
+    assert np.array_equal(got[2, 1][0], got[2, 0][0])
+    assert np.array_equal(got[2, 1][1], got[2, 0][1])
 
 
 @pytest.mark.parametrize("rows_tunable,waves,sweep", [(16, 1, 0), (32, 1, 0), (64, 1, 0), (64, 1, 1), (64, 2, 0),
