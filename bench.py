@@ -412,7 +412,7 @@ def run_ket10(args):
 # --------------------------------------------------------------- dense8192
 def run_dense8192(args):
     import qobhip as Q
-    from qobhip import _lib, workloads as W
+    from qobhip import _lib, tune as T, workloads as W
     from qobhip.core import _dt
     from qobhip.tune import EventTimer
 
@@ -489,9 +489,15 @@ def run_dense8192(args):
         cpu = {"value": n / t, "unit": "applies/s", "cores": blas_threads(), "kind": "port",
                "sample": f"{n} full left applies, numpy restatement of src/operators_lazytensor.jl:281-488 "
                          f"(reshape/permute + BLAS GEMM per mode), ComplexF64, {t:.1f} s"}
+    # with these factors (at most 32 wide) the kernel forms each complex
+    # product from three real MFMAs (Gauss's form) in ComplexF32, and in
+    # ComplexF64 when its factors are resident with one fibre group per wave;
+    # "achieved" counts the algorithmic 8 flops per complex multiply-add
+    gauss = dt == np.complex64 or (T.get("fused_resident") == 1 and T.get("fused_gpw") in (0, 1))
     roof = {"bound": "mfma", "kernel": "k_lt_fused", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
             "frac": tflops / peak, "traffic": traffic_of(f"lt_fused_{args.dtype}"),
-            "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": 2 * N * N * np.dtype(dt).itemsize}
+            "algorithmic_flops_per_launch": flops, "mfma_flops_per_launch": flops * 3 / 4 if gauss else flops,
+            "algorithmic_bytes_per_launch": 2 * N * N * np.dtype(dt).itemsize}
     config = {"workload": f"dense LazyTensor (16, 32, 16) left mul! on an 8192x8192 {args.dtype} density matrix",
               "N": N, "dims": dims, "kernel_ms": {"fused_left": ms_fused, "chain_left": ms_chain,
                                                    "right_tiles": ms_right}, **extra}
