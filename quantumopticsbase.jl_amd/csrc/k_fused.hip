@@ -233,15 +233,13 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
                                                    cplx<R> *__restrict__ out, int64_t ncols, FusedTile TL,
                                                    double2 alpha_, double2 beta_, int beta_zero) {
     using T = cplx<R>;
-    using V = typename mf<R>::v4;
     extern __shared__ unsigned char lds_raw[];
     R *re = (R *)lds_raw;
     const int D = M.D;
     const int Dp = D + (D >> 4);
     R *im = re + Dp;
     T *Al = (T *)(im + Dp + (Dp & 1));  // factor of the current mode, ld = d + 1 (RES: all factors)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
+    const int tid = threadIdx.x;
     const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
 
     // factor of the next mode, loaded into registers one mode ahead so the
