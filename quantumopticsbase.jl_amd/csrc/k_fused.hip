@@ -56,8 +56,7 @@ template <> struct mf<float> {
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 
-constexpr int NTH = 512;   // 8 waves
-constexpr int NW = NTH / 64;
+constexpr int NTH = 512;  // threads per workgroup (8 waves); 1024 for one ComplexF32 shape (see the launcher)
 
 // Each wave owns groups of 16 fibres; for a group it computes every 16-row
 // output tile (ntm = d/16 of them) before writing the group back, so groups
@@ -106,7 +105,7 @@ __device__ __forceinline__ int64_t fused_addr(const FusedTile &T, int D, int64_t
 // back into the planes, or (last mode) to the output column.  DM > 0 fixes
 // the mode size d at compile time; DM = 0 reads it at run time.  M3: three
 // real products per complex k-step (see the file header).
-template <typename R, int MAXNTM, int GPW, bool TILE, int DM, bool M3>
+template <typename R, int MAXNTM, int GPW, bool TILE, int DM, bool M3, int NT>
 __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int ldam, int n1, int d_rt, int D,
                                            bool last, cplx<R> *__restrict__ out, int64_t col, const FusedTile &TL,
                                            cplx<R> alpha, cplx<R> beta, int beta_zero) {
@@ -119,12 +118,12 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
     const int d = DM > 0 ? DM : d_rt;
     const int ntm = d >> 4, ngroups = (D / d) >> 4;
     const int nks = d >> 2;
-    for (int g0 = wave; g0 < ngroups; g0 += NW * GPW) {
+    for (int g0 = wave; g0 < ngroups; g0 += (NT / 64) * GPW) {
         V accr[GPW][NTM], acci[GPW][NTM], accs[GPW][NTM];  // accs: P3 (M3 only)
         int base[GPW];
 #pragma unroll
         for (int q = 0; q < GPW; ++q) {
-            const int g = g0 + q * NW;
+            const int g = g0 + q * (NT / 64);
             const int f = g * 16 + lr;
             const int i1 = f % n1, i3 = f / n1;
             base[q] = i1 + n1 * d * i3;
@@ -141,7 +140,7 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
             R xr[GPW], xi[GPW];
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
-                const int idx = (g0 + q * NW < ngroups) ? pad(base[q] + n1 * (4 * ks + lk)) : 0;
+                const int idx = (g0 + q * (NT / 64) < ngroups) ? pad(base[q] + n1 * (4 * ks + lk)) : 0;
                 xr[q] = re[idx];
                 xi[q] = im[idx];
             }
@@ -181,7 +180,7 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
             // the group's inputs are no longer needed by anyone: write back
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
-                if (g0 + q * NW < ngroups) {
+                if (g0 + q * (NT / 64) < ngroups) {
 #pragma unroll
                     for (int mt = 0; mt < NTM; ++mt) {
                         if (mt < ntm) {
@@ -202,7 +201,7 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
             // while the next column is being computed
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
-                if (g0 + q * NW < ngroups) {
+                if (g0 + q * (NT / 64) < ngroups) {
 #pragma unroll
                     for (int mt = 0; mt < NTM; ++mt) {
                         if (mt < ntm) {
@@ -228,8 +227,8 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
 // prefetch stays in flight across all modes (loads retire in order: a
 // factor load issued after it would have to wait for the whole column), and
 // each mode needs one barrier instead of two.
-template <typename R, int MAXNTM, int EPT, int GPW, bool TILE, bool RES = false>
-__global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *__restrict__ in,
+template <typename R, int MAXNTM, int EPT, int GPW, bool TILE, bool RES = false, int NT = NTH>
+__global__ void __launch_bounds__(NT) k_lt_fused(FusedModes M, const cplx<R> *__restrict__ in,
                                                    cplx<R> *__restrict__ out, int64_t ncols, FusedTile TL,
                                                    double2 alpha_, double2 beta_, int beta_zero) {
     using T = cplx<R>;
@@ -244,7 +243,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
 
     // factor of the next mode, loaded into registers one mode ahead so the
     // load latency hides behind the current mode's MFMAs
-    constexpr int APRE = (16 * MAXNTM) * (16 * MAXNTM) / NTH > 0 ? (16 * MAXNTM) * (16 * MAXNTM) / NTH : 1;
+    constexpr int APRE = (16 * MAXNTM) * (16 * MAXNTM) / NT > 0 ? (16 * MAXNTM) * (16 * MAXNTM) / NT : 1;
     T apre[APRE];
     if constexpr (RES) {
         // ordered before every read of the factors by the first mode's barrier
@@ -252,7 +251,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
         for (int s = 0; s < M.nmodes; ++s) {
             const T *__restrict__ As = (const T *)M.A[s];
             const int dd = M.d[s] * M.d[s];
-            for (int e = tid; e < dd; e += NTH) Al[off + e] = As[e];
+            for (int e = tid; e < dd; e += NT) Al[off + e] = As[e];
             off += dd;
         }
     } else {
@@ -260,7 +259,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
         const int dd = M.d[0] * M.d[0];
 #pragma unroll
         for (int k = 0; k < APRE; ++k) {
-            const int e = tid + k * NTH;
+            const int e = tid + k * NT;
             apre[k] = (e < dd) ? A0[e] : czero<T>();
         }
     }
@@ -270,7 +269,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
     if (col < ncols) {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const int e = tid + k * NTH;
+            const int e = tid + k * NT;
             if constexpr (RES) pre[k] = in[fused_addr<TILE>(TL, D, col, e < D ? e : D - 1)];
             else if (e < D) pre[k] = in[fused_addr<TILE>(TL, D, col, e)];
         }
@@ -278,7 +277,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
     for (; col < ncols; col += gridDim.x) {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const int e = tid + k * NTH;
+            const int e = tid + k * NT;
             if constexpr (RES) {
                 // branch-free, so the wait for pre[k] counts exactly and does
                 // not also wait for the previous column's output stores; lanes
@@ -296,7 +295,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
         if (nxt < ncols) {
 #pragma unroll
             for (int k = 0; k < EPT; ++k) {
-                const int e = tid + k * NTH;
+                const int e = tid + k * NT;
                 if constexpr (RES) {
                     // unconditional (clamped) loads: no branch between them and
                     // their use, so the wait before the use counts exactly
@@ -321,7 +320,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
             if constexpr (!RES) {
 #pragma unroll
             for (int k = 0; k < APRE; ++k) {
-                const int e = tid + k * NTH;
+                const int e = tid + k * NT;
                 if (e < d * d) {
                     const int i = e % d, kk = e / d;
                     Al[i + kk * lda] = apre[k];
@@ -335,7 +334,7 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
                 const int dd = M.d[sn] * M.d[sn];
 #pragma unroll
                 for (int k = 0; k < APRE; ++k) {
-                    const int e = tid + k * NTH;
+                    const int e = tid + k * NT;
                     apre[k] = (e < dd) ? An[e] : czero<T>();
                 }
             }
@@ -353,12 +352,12 @@ __global__ void __launch_bounds__(NTH) k_lt_fused(FusedModes M, const cplx<R> *_
             // against 1.18 ms).
             if constexpr (RES && MAXNTM <= 2 && !(TILE && sizeof(R) == 8 && GPW == 2)) {
                 if (d == 16) {
-                    fused_mode<R, MAXNTM, GPW, TILE, 16, M3>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                    fused_mode<R, MAXNTM, GPW, TILE, 16, M3, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 } else {
-                    fused_mode<R, MAXNTM, GPW, TILE, 32, M3>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                    fused_mode<R, MAXNTM, GPW, TILE, 32, M3, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 }
             } else {
-                fused_mode<R, MAXNTM, GPW, TILE, 0, M3>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                fused_mode<R, MAXNTM, GPW, TILE, 0, M3, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
             }
         }
         __syncthreads();  // every wave is done reading this column's planes
@@ -428,6 +427,12 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
     const bool fits_res = g_fused_resident && fused_lds_res(dt, M) <= 160 * 1024;
     const int gpw = g_fused_gpw ? g_fused_gpw : (dt == QOB_C64 && maxd <= 32 && fits_res ? 1 : 2);
     const bool res = fits_res && gpw <= 2;
+    // automatic shape, ComplexF32 left application of resident factors up to
+    // 32 wide on columns of 4096..8192: 1024 threads with one fibre group per
+    // wave (16 waves of one workgroup share each column's phases); longer
+    // columns would spill at 1024 threads
+    const bool wide = g_fused_gpw == 0 && dt == QOB_C32 && res && tile_ld == 0 && maxd <= 32 && M.D >= 4096 &&
+                      M.D <= 8192;
     const size_t lds = res ? fused_lds_res(dt, M) : fused_lds(dt, M.D, maxd);
     const int ept = (M.D + NTH - 1) / NTH;
     const int ncu = device_cu_count();
@@ -461,6 +466,29 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
                            TL, alpha, beta, bz);                                                             \
         return hipGetLastError();                                                                            \
     } while (0)
+#define QOB_FUW(NTM, EP)                                                                                       \
+    do {                                                                                                     \
+        auto kfn = k_lt_fused<float, NTM, EP, 1, false, true, 1024>;                                         \
+        static std::atomic<int> lds_set{0}; /* raise the limit once per kernel */                            \
+        if (!lds_set.load()) {                                                                               \
+            hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                160 * 1024);                                                 \
+            if (e_ != hipSuccess) return e_;                                                                 \
+            lds_set.store(1);                                                                                \
+        }                                                                                                    \
+        /* persistent grid of the workgroups that are resident at once */                                \
+        static std::atomic<int> nres{0};                                                                     \
+        if (!nres.load()) {                                                                                  \
+            int nb_ = 0;                                                                                     \
+            hipError_t e_ = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_, kfn, 1024, lds);              \
+            if (e_ != hipSuccess) return e_;                                                                 \
+            nres.store(nb_ > 0 ? nb_ : 1);                                                                   \
+        }                                                                                                    \
+        const unsigned gw = (unsigned)std::min<int64_t>(ncols, (int64_t)ncu * nres.load());                 \
+        hipLaunchKernelGGL(kfn, dim3(gw), dim3(1024), lds, s, M, (const cplx<float> *)in, (cplx<float> *)out, \
+                           ncols, TL, alpha, beta, bz);                                                      \
+        return hipGetLastError();                                                                            \
+    } while (0)
 #define QOB_FU(RR, NTM, EP)                                          \
     do {                                                             \
         if (res) {                                                   \
@@ -482,12 +510,14 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
         if (ept <= 8) { if (ntm <= 2) QOB_FU(double, 2, 8); else QOB_FU(double, 4, 8); }
         if (ept <= 16) { if (ntm <= 2) QOB_FU(double, 2, 16); else QOB_FU(double, 4, 16); }
     } else {
+        if (wide) QOB_FUW(2, 8);
         if (ept <= 8) { if (ntm <= 2) QOB_FU(float, 2, 8); else QOB_FU(float, 4, 8); }
         if (ept <= 16) { if (ntm <= 2) QOB_FU(float, 2, 16); else QOB_FU(float, 4, 16); }
         if (ept <= 32) { if (ntm <= 2) QOB_FU(float, 2, 32); else QOB_FU(float, 4, 32); }
     }
     return hipErrorInvalidValue;
 #undef QOB_FU
+#undef QOB_FUW
 #undef QOB_FU3R
 #undef QOB_FU3
     return hipGetLastError();

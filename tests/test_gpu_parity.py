@@ -429,7 +429,8 @@ def test_fused_resident_factors(shape, idx, dtype, tol):
     staging, with two fibre groups per wave in both: bitwise equal (same
     products in the same order), left on columns and right on 16-row tiles.
     The automatic launch shape (ComplexF64: one fibre group per wave and
-    Gauss's three-product form when resident) against the oracle."""
+    Gauss's three-product form when resident; ComplexF32 at D = 8192: 1024
+    threads) against the oracle."""
     g = np.random.default_rng(90 + sum(shape))
     lt = rand_lt(g, shape, shape, idx, ("dense",) * len(idx))
     D = int(np.prod(shape))
@@ -459,6 +460,11 @@ def test_fused_resident_factors(shape, idx, dtype, tol):
         assert rel(got[key][1], want_r) < tol
     assert np.array_equal(got[2, 1][0], got[2, 0][0])
     assert np.array_equal(got[2, 1][1], got[2, 0][1])
+    if dtype == np.complex64:
+        # ComplexF32 runs the same three-product tiles in every shape (the
+        # automatic one is 1024 threads, one group per wave at D = 8192)
+        assert np.array_equal(got[0, 1][0], got[2, 1][0])
+        assert np.array_equal(got[0, 1][1], got[2, 1][1])
 
 
 @pytest.mark.parametrize("rows_tunable,waves,sweep", [(16, 1, 0), (32, 1, 0), (64, 1, 0), (64, 1, 1), (64, 2, 0),
