@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(512) k_modek(const T *__restrict__ A, int d_ou
 
 }  // namespace
 
-int g_modek_nth = 256;  // fibres (lanes) per workgroup: 64, 128, 256 or 512
+int g_modek_nth = 64;  // fibres (lanes) per workgroup: 64, 128, 256 or 512 (64 measured fastest)
 
 bool modek_supported(int64_t d_out, int64_t d_in) { return d_in <= 64 && d_out >= 1 && d_in >= 1; }
 

@@ -26,3 +26,15 @@ def gpu_available() -> bool:
 @pytest.fixture(scope="session", autouse=True)
 def _announce(request):
     yield
+
+
+@pytest.fixture(autouse=True)
+def _gpu_context(request):
+    """GPU tests run with the library context already created.  Creating it
+    applies the packaged launch-shape file, so a test that sets tunables
+    before its first library call would otherwise have them reset by that
+    call when it happens to be the first GPU test in the process."""
+    if request.node.get_closest_marker("gpu") is not None:
+        import qobhip
+        qobhip.context()
+    yield

@@ -467,6 +467,39 @@ def test_fused_resident_factors(shape, idx, dtype, tol):
         assert np.array_equal(got[0, 1][1], got[2, 1][1])
 
 
+@pytest.mark.parametrize("gpw", [0, 1, 2, 4])
+@pytest.mark.parametrize("resident", [1, 0])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
+def test_fused_every_launch_shape(gpw, resident, dtype, tol):
+    """Every fibre-groups-per-wave setting with and without resident
+    factors (four or three real products per complex k-step, 512 or 1024
+    threads) on the cfg3 factor shape, left on columns and right on tiles,
+    against the oracle."""
+    g = np.random.default_rng(95)
+    shape, idx = (16, 32, 16), (1, 2, 3)
+    lt = rand_lt(g, shape, shape, idx, ("dense",) * 3)
+    D = 8192
+    B = cg(g, D, 3)
+    Bt = (g.standard_normal((64, D)) + 1j * g.standard_normal((64, D))) / np.sqrt(D)  # 64 rows: tiled path
+    want = np.zeros((D, 3), complex)
+    O.mul_op_left(want, lt, B, 1.0, 0.0)
+    want_r = np.zeros((64, D), complex)
+    O.mul_op_right(want_r, Bt, lt, 0.5j, 0.0)
+    try:
+        _tune("fused_gpw", gpw)
+        _tune("fused_resident", resident)
+        q = to_q(lt)
+        R = Q.Operator(q.basis_l, gb(3), np.full((D, 3), np.nan, complex), dtype)
+        Q.mul_(R, q, Q.Operator(q.basis_r, gb(3), B, dtype), 1.0, 0.0)
+        Rt = Q.Operator(gb(64), q.basis_r, np.full((64, D), np.nan, complex), dtype)
+        Q.mul_(Rt, Q.Operator(gb(64), q.basis_l, Bt, dtype), q, 0.5j, 0.0)
+    finally:
+        _tune("fused_resident", 1)
+        _tune("fused_gpw", 0)
+    assert rel(R.to_numpy(), want) < tol
+    assert rel(Rt.to_numpy(), want_r) < tol
+
+
 @pytest.mark.parametrize("rows_tunable,waves,sweep", [(16, 1, 0), (32, 1, 0), (64, 1, 0), (64, 1, 1), (64, 2, 0),
                                                       (64, 4, 1), (64, 4, 2)])
 @pytest.mark.parametrize("cc", [1, 3])
