@@ -288,6 +288,59 @@ __global__ void __launch_bounds__(64) k_csc_right_groups(int64_t rows, const int
     }
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// The same for ComplexF32 with an even row count and 16-byte aligned B and
+// R: each lane owns two adjacent rows, so every load and store moves 16
+// bytes (a wave covers 128 rows).  Each row's arithmetic is the one above,
+// in the same order.
+template <int GM>
+__global__ void __launch_bounds__(64) k_csc_right_groups_x2(int64_t rows, const int32_t *__restrict__ gint,
+                                                            const float2 *__restrict__ gcoef,
+                                                            const float2 *__restrict__ B, float2 *__restrict__ R,
+                                                            double2 alpha_, double2 beta_, int beta_zero,
+                                                            uint32_t nrb, int remap) {
+    constexpr int NI = 4 + 2 * GM;
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
+    const uint32_t g = lb / nrb, rb = lb % nrb;
+    const int32_t *gi = gint + (int64_t)g * NI;
+    const float2 *gc = gcoef + (int64_t)g * GM * GM;
+    const int nout = gi[0], nin = gi[1];
+    const uint64_t mask = (uint64_t)(uint32_t)gi[2] | ((uint64_t)(uint32_t)gi[3] << 32);
+    const int64_t i = (int64_t)rb * 128 + 2 * threadIdx.x;  // rows i and i + 1 (rows is even)
+    const bool ok = i < rows;
+    const float2 alpha = to_t<float2>(alpha_), beta = to_t<float2>(beta_);
+    float2 b0[GM], b1[GM];
+#pragma unroll
+    for (int q = 0; q < GM; ++q) {
+        f4v v = {0.f, 0.f, 0.f, 0.f};
+        if (q < nin && ok) v = *(const f4v *)(B + i + (int64_t)gi[4 + GM + q] * rows);
+        b0[q] = make_float2(v.x, v.y);
+        b1[q] = make_float2(v.z, v.w);
+    }
+#pragma unroll
+    for (int o = 0; o < GM; ++o) {
+        if (o < nout) {
+            float2 a0 = czero<float2>(), a1 = czero<float2>();
+#pragma unroll
+            for (int q = 0; q < GM; ++q)
+                if ((mask >> (o * GM + q)) & 1ull) {
+                    a0 = cfma(gc[o * GM + q], b0[q], a0);
+                    a1 = cfma(gc[o * GM + q], b1[q], a1);
+                }
+            if (ok) {
+                float2 *out = R + i + (int64_t)gi[4 + o] * rows;
+                const float2 r0 = axpby(alpha, a0, beta, out, (beta_zero & 1) != 0);
+                const float2 r1 = axpby(alpha, a1, beta, out + 1, (beta_zero & 1) != 0);
+                const f4v v = {r0.x, r0.y, r1.x, r1.y};
+                if (beta_zero & 2) __builtin_nontemporal_store(v, (f4v *)out);
+                else *(f4v *)out = v;
+            }
+        }
+    }
+}
+
 // ---- right, short stripes: a wave covers RR rows x (64 / RR) columns per
 // step (lane l: row l % RR, column l / RR).  Workgroups of one XCD sweep a
 // stripe of RR rows across all columns, so a column of B re-read by the
@@ -515,6 +568,18 @@ hipError_t launch_csc_right(qob_dtype dt, const CscDev &X, const void *B, int64_
             if (X.gmax == 2) QOB_CG(double2, 2);
             else if (X.gmax == 4) QOB_CG(double2, 4);
             else QOB_CG(double2, 8);
+        } else if (rows % 2 == 0 && (((uintptr_t)B | (uintptr_t)R) & 15) == 0) {
+            // two rows per lane: 128-row blocks
+            const uint64_t nrb2 = (uint64_t)((rows + 127) / 128);
+            const uint64_t G2 = nrb2 * (uint64_t)X.ngroups;
+#define QOB_CG2(GMV)                                                                                           \
+    hipLaunchKernelGGL((k_csc_right_groups_x2<GMV>), dim3((uint32_t)G2), dim3(64), 0, s, rows, X.gint,         \
+                       (const float2 *)X.gcoef, (const float2 *)B, (float2 *)R, alpha, beta, bz, (uint32_t)nrb2, \
+                       g_csc_gremap)
+            if (X.gmax == 2) QOB_CG2(2);
+            else if (X.gmax == 4) QOB_CG2(4);
+            else QOB_CG2(8);
+#undef QOB_CG2
         } else {
             if (X.gmax == 2) QOB_CG(float2, 2);
             else if (X.gmax == 4) QOB_CG(float2, 4);

@@ -714,17 +714,20 @@ def test_sparse_left_closed_slices(sizes, adj, dtype, tol):
 @pytest.mark.parametrize("sizes", [[1, 2, 3, 4] * 50, [8, 7, 1, 5] * 20, [2] * 60, [9] + [3] * 40])
 @pytest.mark.parametrize("adj", [False, True])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
-def test_sparse_right_component_groups(sizes, adj, dtype, tol):
+@pytest.mark.parametrize("rows", [70, 133, 200])
+def test_sparse_right_component_groups(sizes, adj, dtype, tol, rows):
     """rho * X for block-structured X: components of at most 8 columns run
     as groups (one load of each column of rho per row block); a 9-column
-    component leaves the table without groups.  Against the oracle with
+    component leaves the table without groups.  ComplexF32 with an even row
+    count runs two rows per lane (70: one partly filled 128-row block, 200:
+    two blocks); 133 rows keep one row per lane.  Against the oracle with
     beta = 0 over NaN and beta != 0, and bit for bit against the CSC
     kernel."""
     import ctypes as C
     from qobhip import _lib
     g = np.random.default_rng(sum(sizes) + 3 * len(sizes))
     X = _block_matrix(g, sizes, n_extra_isolated=3)
-    n, rows = X.shape[0], 70
+    n = X.shape[0]
     oop = O.OSparse.from_scipy(X)
     if adj:
         oop = O.dagger(oop)
