@@ -292,6 +292,16 @@ qob_status qob_trace(qob_ctx ctx, qob_dtype dt, int64_t n, const void *A,
  * (src/operators_dense.jl:217), stream-ordered, no host round trip. */
 qob_status qob_normalize_ket(qob_ctx ctx, qob_dtype dt, int64_t n, void *x, qob_stream s);
 qob_status qob_normalize_op(qob_ctx ctx, qob_dtype dt, int64_t n, void *A, qob_stream s);
+/* normalize!(ket) exactly as qob_normalize_ket (same kernels, bitwise the
+ * same result), but allocating nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The norm's partial sums and the norm live
+ * in a fixed 16 KiB + 16 B device array per GPU, held until the process
+ * ends; every n fits it (the partial count is capped at 1024).  Calls that
+ * overlap on one GPU share that array and race: issue them on one stream,
+ * or order them with an event.  The device's array is looked up when a
+ * context is created on it. */
+qob_status qob_normalize_ket_noalloc(qob_ctx ctx, qob_dtype dt, int64_t n, void *x, qob_stream s);
 /* expect(op, ket) = <psi|op|psi> (src/operators_dense.jl:219-221,
  * src/operators.jl:119) and expect(op, rho) = tr(op * rho)
  * (src/operators_dense.jl:223-230, src/operators_sparse.jl:36-45).  In the

@@ -40,13 +40,26 @@ extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
 
-static qob_status fail(qob_status st, const char *fmt, ...) {
+static qob_status vfail(qob_status st, const char *fmt, va_list ap) {
     char buf[512];
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    g_err = buf;
+    return st;
+}
+
+static qob_status fail(qob_status st, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    vfail(st, fmt, ap);
     va_end(ap);
-    g_err = buf;
+    return st;
+}
+
+qob_status set_error(qob_status st, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vfail(st, fmt, ap);
+    va_end(ap);
     return st;
 }
 
@@ -813,6 +826,8 @@ static qob_status ws_get(qob_ctx ctx, hipStream_t s, int slot, size_t bytes, voi
 }
 
 static hipStream_t strm(qob_ctx ctx, qob_stream s) { return s ? (hipStream_t)s : ctx->stream; }
+hipStream_t ctx_stream(qob_ctx ctx, qob_stream s) { return strm(ctx, s); }
+int ctx_device(qob_ctx ctx) { return ctx->device; }
 
 // ------------------------------------------------------------ plan building
 struct TermSpec {
@@ -2070,6 +2085,7 @@ qob_status qob_ctx_create(int device, qob_ctx *out) {
     QOB_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(QOB_EARG, "device %d not present (%d devices)", device, n);
     QOB_HIP(hipSetDevice(device));
+    QOB_TRY(noalloc_init(device));
     auto *c = new qob_ctx_s();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -273,4 +273,14 @@ hipError_t launch_qubit_dot(qob_dtype dt, QubitStateDev single, const void *u, i
 int csc_trace_blocks(int64_t n);
 hipError_t launch_csc_trace(qob_dtype dt, const CscDev &X, const void *V, int64_t n, double2 *partials, hipStream_t s);
 
+// ------------------------------------------------- host-side shared helpers
+// Record the text of a failing call for qob_last_error() and return st.
+qob_status set_error(qob_status st, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// Device index of a context and the stream a call runs on (s, or the
+// context's own stream when s is null).
+int ctx_device(qob_ctx ctx);
+hipStream_t ctx_stream(qob_ctx ctx, qob_stream s);
+// Look up the no-allocation scratch array's address on `device` (current).
+qob_status noalloc_init(int device);
+
 }  // namespace qob

@@ -115,6 +115,7 @@ _PROTOS = {
     "qob_trace": (_i32, [_vp, _i32, _i64, _vp, _P(qob_c64), _vp]),
     "qob_normalize_ket": (_i32, [_vp, _i32, _i64, _vp, _vp]),
     "qob_normalize_op": (_i32, [_vp, _i32, _i64, _vp, _vp]),
+    "qob_normalize_ket_noalloc": (_i32, [_vp, _i32, _i64, _vp, _vp]),
     "qob_expect_ket": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
     "qob_expect_op": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
     "qob_variance_ket": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
