@@ -1,18 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the mul! hot path on one MI355X.  Prints ONE JSON line.
 
-Default workload (--workload sparse6400), the ComplexF64 density-matrix
-configuration:
-  H = a^dag a (x) 1 (x) 1 + 1 (x) b^dag b (x) 1 + g (a (x) b^dag (x) s+ + h.c.)
-  on Fock(39) (x) Fock(39) (x) Spin(3/2): N = 6400, nnz = 15522, g ~ U[0,1];
-  rho = G G^dag / tr, G complex Gaussian 6400 x 6400 (seeded);
-  one step = mul!(out, H, rho, 1, 0) + mul!(out2, rho, H, 1, 0) = 2 applies.
+Default workload (--workload dense8192), the largest single-GPU
+configuration: a LazyTensor of three dense random Hermitian factors
+(dims 16, 32, 16; entries complex Gaussian, symmetrised) applied from the
+left to an 8192 x 8192 Ginibre trace-1 density matrix in ComplexF64
+(--dtype c32 for the ComplexF32 FP32-MFMA form); one step = one
+mul!(out, LT, rho, 1, 0).  vs_baseline is the ratio to the plain per-call
+GPU path on the same workload (the mode-by-mode chain).  The line also
+carries the 6400 x 6400 SparseOperator figures under config.sparse6400.
 
 Other workloads (--workload):
+  sparse6400 SparseOperator on Fock(39) (x) Fock(39) (x) Spin(3/2), N = 6400,
+             nnz = 15522, mul! from the left and the right onto a Ginibre
+             trace-1 density matrix; one step = 2 applies
   ket10      10-site spin chain, H = sum J sx sx + sum h sz (19 LazyTensors),
              mul!(out, H, psi, 1, 0) on a normalised complex-Gaussian Ket
-  dense8192  LazyTensor of dense Hermitian factors (16, 32, 16) on an
-             8192 x 8192 density matrix, left mul!, ComplexF64 or ComplexF32
   batch256   256 transverse-field Ising chains of 8..20 sites, one launch per
              step x' = x + c H x for all of them
   serve      Poisson request stream of mixed jobs (32 x (mul! + axpy) +
@@ -410,6 +413,69 @@ def run_ket10(args):
 
 
 # --------------------------------------------------------------- dense8192
+def timed_events(ctx, step, steps: int, warmup: int):
+    """W untimed warmup steps, then exactly K steps between synchronisations.
+    An event on the library's stream before the first timed step and after
+    each one gives the kernel time over the timed region itself.  Returns
+    (host seconds, event ms over the K steps, list of K per-step event ms)."""
+    from qobhip import _lib
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    evs = []
+    for _ in range(steps + 1):
+        e = C.c_void_p()
+        _lib.call("qob_event_create", ctx.h, C.byref(e))
+        evs.append(e)
+    _lib.call("qob_event_record", ctx.h, evs[0], ctx.stream)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step()
+        _lib.call("qob_event_record", ctx.h, evs[i + 1], ctx.stream)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+
+    def ms(a, b):
+        v = C.c_float()
+        _lib.call("qob_event_elapsed", a, b, C.byref(v))
+        return float(v.value)
+
+    total = ms(evs[0], evs[-1])
+    per = [ms(evs[i], evs[i + 1]) for i in range(steps)]
+    for e in evs:
+        _lib.load().qob_event_destroy(e)
+    return el, total, per
+
+
+def sparse6400_figures(steps: int = 20):
+    """The 6400x6400 SparseOperator density-matrix workload, kernels only:
+    left and right apply times from events, applies/s of a left + right step,
+    and the dominant kernel's fraction of HBM peak."""
+    import qobhip as Q
+    from qobhip import _lib, workloads as W
+    from qobhip.tune import EventTimer
+
+    spec = W.fock_fock_spin(seed=1)
+    n, nnz = spec.H.shape[0], spec.H.nnz
+    H = Q.SparseOperator(spec.basis, spec.basis, spec.H)
+    rho = W.ginibre_rho_device(seed=2, n=n)
+    outL = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
+    outR = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n))
+    ctx = Q.context()
+    left = lambda: Q.mul_(outL, H, rho, 1.0, 0.0)
+    right = lambda: Q.mul_(outR, rho, H, 1.0, 0.0)
+    el = timed(lambda: (left(), right()), steps, 3, ctx.synchronize)
+    ev = EventTimer(ctx.h, ctx.stream)
+    ms_l, ms_r = ev(left, steps), ev(right, steps)
+    ms_copy = ev(lambda: _lib.call("qob_copy", ctx.h, 0, n * n, rho.data.ptr, outL.data.ptr, None), steps)
+    b = 32 * n * n + nnz * 20 + (n + 1) * 4
+    return {"workload": "SparseOperator mul! left + right on a 6400x6400 ComplexF64 density matrix "
+                        "(Fock(39) x Fock(39) x Spin(3/2), nnz 15522)",
+            "applies_per_s": 2 * steps / el, "kernel_ms": {"left": ms_l, "right": ms_r, "copy_same_bytes": ms_copy},
+            "algorithmic_bytes_per_launch": b,
+            "hbm_frac": {"left": b / (ms_l * 1e-3) / 1e9 / HBM_PEAK_GBS, "right": b / (ms_r * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+
+
 def run_dense8192(args):
     import qobhip as Q
     from qobhip import _lib, tune as T, workloads as W
@@ -438,17 +504,22 @@ def run_dense8192(args):
     if args.probe:
         return probe(ctx, rho.data, out.data, left)
     log(f"bench: dense8192 {args.dtype} warmup {args.warmup}, timing {args.steps} steps")
-    el = timed(left, args.steps, args.warmup, ctx.synchronize)
+    # one step is one left apply, which is one launch of the fused kernel:
+    # the events around the timed steps give that kernel's average duration
+    el, ev_ms, per = timed_events(ctx, left, args.steps, args.warmup)
+    ms_fused = ev_ms / args.steps
     ev = EventTimer(ctx.h, ctx.stream)
-    ms_fused = ev(left, 10)
     right()
     ctx.synchronize()
     ms_right = ev(right, 5)
+    # the plain per-call path: the reference's mode-by-mode chain
+    # (_tp_sum_matmul!, one pass over the state per factor), same stream
     _lib.call("qob_tune_set", b"fused_lt", 0)
-    left()
-    ctx.synchronize()
-    ms_chain = ev(left, 5)
+    plain_steps = max(3, min(10, args.steps))
+    el_plain, ev_plain, _ = timed_events(ctx, left, plain_steps, 1)
+    ms_chain = ev_plain / plain_steps
     _lib.call("qob_tune_set", b"fused_lt", 1)
+    plain_aps = plain_steps / el_plain
     flops = 8.0 * N * N * sum(dims)
     peak = FP32_MFMA_TFLOPS if dt == np.complex64 else FP64_MFMA_TFLOPS
     tflops = flops / (ms_fused * 1e-3) / 1e12
@@ -472,7 +543,8 @@ def run_dense8192(args):
         del out64, up, diff
     cpu = None
     if not args.no_cpu:
-        log("bench: CPU baseline (numpy restatement of _tp_sum_matmul!, BLAS threads)")
+        nthr = blas_threads()
+        log(f"bench: CPU baseline (numpy restatement of _tp_sum_matmul!, {nthr} BLAS threads)")
         sys.path.insert(0, str(ROOT / "oracle"))
         import qob_oracle as O
         rho_h = rho64.to_numpy()
@@ -486,24 +558,43 @@ def run_dense8192(args):
             t = time.perf_counter() - t0
             if t > 10.0:
                 break
-        cpu = {"value": n / t, "unit": "applies/s", "cores": blas_threads(), "kind": "port",
-               "sample": f"{n} full left applies, numpy restatement of src/operators_lazytensor.jl:281-488 "
-                         f"(reshape/permute + BLAS GEMM per mode), ComplexF64, {t:.1f} s"}
+        cpu = {"value": n / t, "unit": "applies/s", "cores": nthr, "kind": "port",
+               "sample": f"{n} full left applies of the same 8192x8192 workload, numpy restatement of "
+                         f"src/operators_lazytensor.jl:281-488 (reshape/permute + BLAS GEMM per mode), ComplexF64, "
+                         f"numpy BLAS on {nthr} threads, {t:.1f} s"}
+        del rho_h, res
     # with these factors (at most 32 wide) the kernel forms each complex
     # product from three real MFMAs (Gauss's form) in ComplexF32, and in
     # ComplexF64 when its factors are resident with one fibre group per wave;
     # "achieved" counts the algorithmic 8 flops per complex multiply-add
     gauss = dt == np.complex64 or (T.get("fused_resident") == 1 and T.get("fused_gpw") in (0, 1))
+    mfma_flops = flops * 3 / 4 if gauss else flops
     roof = {"bound": "mfma", "kernel": "k_lt_fused", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
             "frac": tflops / peak, "traffic": traffic_of(f"lt_fused_{args.dtype}"),
-            "algorithmic_flops_per_launch": flops, "mfma_flops_per_launch": flops * 3 / 4 if gauss else flops,
+            "algorithmic_flops_per_launch": flops, "mfma_flops_per_launch": mfma_flops,
+            "executed_tflops": mfma_flops / (ms_fused * 1e-3) / 1e12,
+            "executed_frac": mfma_flops / (ms_fused * 1e-3) / 1e12 / peak,
+            "kernel_ms_avg": ms_fused, "timing": "events on the library stream around the timed steps",
             "algorithmic_bytes_per_launch": 2 * N * N * np.dtype(dt).itemsize}
-    config = {"workload": f"dense LazyTensor (16, 32, 16) left mul! on an 8192x8192 {args.dtype} density matrix",
-              "N": N, "dims": dims, "kernel_ms": {"fused_left": ms_fused, "chain_left": ms_chain,
-                                                   "right_tiles": ms_right}, **extra}
-    return base_result(args, args.steps / el, "applies/s", el * 1e3 / args.steps,
-                       "f32" if dt == np.complex64 else "f64",
-                       "synthetic (seeded Hermitian factors, Ginibre trace-1 rho)", config, roof, cpu)
+    ps = np.percentile(per, [50, 99])
+    config = {"workload": f"config 3: dense LazyTensor (16, 32, 16) of Hermitian factors, left mul! on an "
+                          f"8192x8192 {args.dtype} Ginibre trace-1 density matrix; 1 step = 1 apply",
+              "N": N, "dims": dims,
+              "kernel_ms": {"fused_left": ms_fused, "chain_left": ms_chain, "right_tiles": ms_right},
+              "latency_ms_per_apply": {"p50": float(ps[0]), "p99": float(ps[1])},
+              "plain_gpu_path": {"applies_per_s": plain_aps,
+                                 "what": "mode-by-mode chain on the GPU (_tp_sum_matmul!, one pass per factor)"},
+              **extra}
+    del out, outR, rho, rho64, LT
+    if not args.no_plain:
+        log("bench: 6400x6400 sparse figures")
+        config["sparse6400"] = sparse6400_figures()
+    res = base_result(args, args.steps / el, "applies/s", el * 1e3 / args.steps,
+                      "f32" if dt == np.complex64 else "f64",
+                      "synthetic (seeded Hermitian factors, Ginibre trace-1 rho)", config, roof, cpu)
+    res["vs_baseline"] = (args.steps / el) / plain_aps
+    res["vs_baseline_what"] = "value / applies/s of the plain per-call GPU path (mode-by-mode chain), same workload"
+    return res
 
 
 # ---------------------------------------------------------------- batch256
@@ -649,7 +740,7 @@ def run_serve(args):
 
 WORKLOADS = {"sparse6400": run_sparse6400, "ket10": run_ket10, "dense8192": run_dense8192,
              "batch256": run_batch256, "serve": run_serve}
-DEFAULT_STEPS = {"sparse6400": (200, 10), "ket10": (2000, 100), "dense8192": (20, 3), "batch256": (200, 10),
+DEFAULT_STEPS = {"sparse6400": (200, 10), "ket10": (2000, 100), "dense8192": (40, 5), "batch256": (200, 10),
                  "serve": (0, 0)}
 
 
@@ -658,7 +749,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="sparse6400")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dense8192")
     ap.add_argument("--dtype", choices=["c64", "c32", "both"], default=None)
     ap.add_argument("--requests", type=int, default=48, help="serve: requests per arrival rate")
     ap.add_argument("--budget-gb", type=int, default=64, help="serve: HBM budget for admitted jobs")

@@ -13,11 +13,17 @@
 //      Y = A x_k X is a GEMM  (d x d) * (d x D/d): 16 x 16 output tiles on
 //      v_mfma_{f64,f32}_16x16x4, accumulators in registers; all waves
 //      finish reading before any tile is written back into the planes.
-//      Four real MFMAs per complex k-step (re*re - im*im, re*im + im*re),
-//      or three where the kernel has room for a third accumulator (Gauss's
-//      product: P1 = Ar Xr, P2 = Ai Xi, P3 = (Ar + Ai)(Xr + Xi);
-//      Re = P1 - P2, Im = P3 - P1 - P2): ComplexF32 with two fibre groups
-//      per wave, ComplexF64 with one;
+//      ComplexF64: four real MFMAs per complex k-step, as BLAS forms a
+//      complex product, so each of the real and imaginary parts is as
+//      accurate as the reference's; with one fibre group per wave the four
+//      products go to four accumulators (Ar Xr, Ai Xi, Ar Xi, Ai Xr: four
+//      independent MFMA chains; Re = first - second, Im = third + fourth at
+//      the end), else to two (re*re - im*im, re*im + im*re).  ComplexF32:
+//      three real MFMAs where the kernel has room for a third accumulator
+//      (Gauss's product: P1 = Ar Xr, P2 = Ai Xi, P3 = (Ar + Ai)(Xr + Xi);
+//      Re = P1 - P2, Im = P3 - P1 - P2; its error bound is normwise, the
+//      imaginary part loses accuracy where it is much smaller than the real
+//      part), else four;
 //   3. the column is written once: out = alpha * Y + beta * out.
 // ComplexF64 uses FP64 MFMA; ComplexF32 uses FP32 MFMA with FP32
 // accumulation.  Columns are distributed over persistent workgroups.
@@ -103,9 +109,12 @@ __device__ __forceinline__ int64_t fused_addr(const FusedTile &T, int D, int64_t
 // One mode of one column: each wave takes groups of 16 fibres (GPW at a
 // time), forms every 16-row output tile of the group on MFMA and writes it
 // back into the planes, or (last mode) to the output column.  DM > 0 fixes
-// the mode size d at compile time; DM = 0 reads it at run time.  M3: three
-// real products per complex k-step (see the file header).
-template <typename R, int MAXNTM, int GPW, bool TILE, int DM, bool M3, int NT>
+// the mode size d at compile time; DM = 0 reads it at run time.  CM: how a
+// complex k-step is formed (see the file header): CM_TWO four real products
+// into two accumulators, CM_GAUSS three products, CM_FOUR four products
+// into four accumulators.
+enum { CM_TWO = 0, CM_GAUSS = 1, CM_FOUR = 2 };
+template <typename R, int MAXNTM, int GPW, bool TILE, int DM, int CM, int NT>
 __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int ldam, int n1, int d_rt, int D,
                                            bool last, cplx<R> *__restrict__ out, int64_t col, const FusedTile &TL,
                                            cplx<R> alpha, cplx<R> beta, int beta_zero) {
@@ -119,7 +128,9 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
     const int ntm = d >> 4, ngroups = (D / d) >> 4;
     const int nks = d >> 2;
     for (int g0 = wave; g0 < ngroups; g0 += (NT / 64) * GPW) {
-        V accr[GPW][NTM], acci[GPW][NTM], accs[GPW][NTM];  // accs: P3 (M3 only)
+        // CM_GAUSS: accs = P3; CM_FOUR: accr = Ar Xr, acci = Ar Xi,
+        // accs = Ai Xi, acct = Ai Xr
+        V accr[GPW][NTM], acci[GPW][NTM], accs[GPW][NTM], acct[GPW][NTM];
         int base[GPW];
 #pragma unroll
         for (int q = 0; q < GPW; ++q) {
@@ -132,6 +143,7 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
                 accr[q][mt] = V{0, 0, 0, 0};
                 acci[q][mt] = V{0, 0, 0, 0};
                 accs[q][mt] = V{0, 0, 0, 0};
+                acct[q][mt] = V{0, 0, 0, 0};
             }
         }
 #pragma unroll
@@ -150,10 +162,15 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
                     const T a = Am[(mt * 16 + lr) + (4 * ks + lk) * ldam];
 #pragma unroll
                     for (int q = 0; q < GPW; ++q) {
-                        if constexpr (M3) {
+                        if constexpr (CM == CM_GAUSS) {
                             accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
                             acci[q][mt] = mf<R>::mma(a.y, xi[q], acci[q][mt]);
                             accs[q][mt] = mf<R>::mma(a.x + a.y, xr[q] + xi[q], accs[q][mt]);
+                        } else if constexpr (CM == CM_FOUR) {
+                            accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
+                            acci[q][mt] = mf<R>::mma(a.x, xi[q], acci[q][mt]);
+                            accs[q][mt] = mf<R>::mma(a.y, xi[q], accs[q][mt]);
+                            acct[q][mt] = mf<R>::mma(a.y, xr[q], acct[q][mt]);
                         } else {
                             accr[q][mt] = mf<R>::mma(a.x, xr[q], accr[q][mt]);
                             accr[q][mt] = mf<R>::mma(a.y, -xi[q], accr[q][mt]);
@@ -164,7 +181,7 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
                 }
             }
         }
-        if constexpr (M3) {
+        if constexpr (CM == CM_GAUSS) {
             // (P1, P2, P3) -> (Re, Im)
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
@@ -173,6 +190,15 @@ __device__ __forceinline__ void fused_mode(R *re, R *im, const cplx<R> *Am, int 
                     const V p1 = accr[q][mt], p2 = acci[q][mt];
                     accr[q][mt] = p1 - p2;
                     acci[q][mt] = accs[q][mt] - p1 - p2;
+                }
+            }
+        } else if constexpr (CM == CM_FOUR) {
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) {
+#pragma unroll
+                for (int mt = 0; mt < NTM; ++mt) {
+                    accr[q][mt] = accr[q][mt] - accs[q][mt];
+                    acci[q][mt] = acci[q][mt] + acct[q][mt];
                 }
             }
         }
@@ -307,12 +333,14 @@ __global__ void __launch_bounds__(NT) k_lt_fused(FusedModes M, const cplx<R> *__
         }
         int aoff = 0;  // RES: offset of the current mode's factor
 
-        // Gauss's product where its third accumulator adds no register
-        // spills: FP32 with factors up to 32 wide, up to 16 elements per
-        // lane and up to two fibre groups per wave; FP64 with resident
-        // factors up to 32 wide and one fibre group per wave
-        constexpr bool M3 = (sizeof(R) == 4 && MAXNTM <= 2 && EPT <= 16 && GPW <= 2) ||
-                            (sizeof(R) == 8 && MAXNTM <= 2 && GPW == 1 && RES);
+        // FP32: Gauss's product where its third accumulator adds no register
+        // spills (factors up to 32 wide, up to 16 elements per lane, up to
+        // two fibre groups per wave).  FP64: four products, into four
+        // accumulators with resident factors up to 32 wide and one fibre
+        // group per wave
+        constexpr int CM = (sizeof(R) == 4 && MAXNTM <= 2 && EPT <= 16 && GPW <= 2) ? CM_GAUSS
+                           : (sizeof(R) == 8 && MAXNTM <= 2 && GPW == 1 && RES)   ? CM_FOUR
+                                                                                  : CM_TWO;
         for (int s = 0; s < M.nmodes; ++s) {
             const int n1 = RES ? mode_sel(M.n1, s) : M.n1[s], d = RES ? mode_sel(M.d, s) : M.d[s];
             const int lda = d + 1;
@@ -352,12 +380,12 @@ __global__ void __launch_bounds__(NT) k_lt_fused(FusedModes M, const cplx<R> *__
             // against 1.18 ms).
             if constexpr (RES && MAXNTM <= 2 && !(TILE && sizeof(R) == 8 && GPW == 2)) {
                 if (d == 16) {
-                    fused_mode<R, MAXNTM, GPW, TILE, 16, M3, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                    fused_mode<R, MAXNTM, GPW, TILE, 16, CM, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 } else {
-                    fused_mode<R, MAXNTM, GPW, TILE, 32, M3, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                    fused_mode<R, MAXNTM, GPW, TILE, 32, CM, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
                 }
             } else {
-                fused_mode<R, MAXNTM, GPW, TILE, 0, M3, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
+                fused_mode<R, MAXNTM, GPW, TILE, 0, CM, NT>(re, im, Am, ldam, n1, d, D, last, out, col, TL, alpha, beta, beta_zero);
             }
         }
         __syncthreads();  // every wave is done reading this column's planes
@@ -367,7 +395,7 @@ __global__ void __launch_bounds__(NT) k_lt_fused(FusedModes M, const cplx<R> *__
 }  // namespace
 
 // fibre groups in flight per wave: 1, 2 or 4; 0 (automatic): one for
-// ComplexF64 with resident factors up to 32 wide (Gauss's product), else two
+// ComplexF64 with resident factors up to 32 wide (four accumulators), else two
 int g_fused_gpw = 0;
 int g_fused_tiles = 1;     // 1: right applications run the fused kernel on 16-row tiles
 int g_fused_resident = 1;  // 1: factors resident in LDS when planes + factors fit (two fibre groups per wave)

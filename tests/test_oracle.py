@@ -442,6 +442,62 @@ def test_c_lazysum_engine_matches_oracle_and_kron():
     assert np.array_equal(r, np.zeros(D))
 
 
+@pytest.mark.parametrize("shape,seed", [((2, 3, 2, 4), 5), ((3, 2, 2, 2, 3), 6), ((2,) * 9, 7), ((5,), 8)])
+def test_c_flat_lazysum_engine_bit_equal_to_recursion(shape, seed):
+    """qobref_lazysum_ket_flat (each term's recursion unrolled into nested
+    loops; the checker of long step sequences at full size) equals the
+    recursive engine bit for bit, with factors at the first and last sites,
+    several stored entries per column and empty terms' identity sites; and
+    both equal the Kronecker product."""
+    import scipy.sparse as sp
+    from cref import LazySumKet
+    g = np.random.default_rng(seed)
+    n = len(shape)
+    D = int(np.prod(shape))
+    terms = []
+    for t in range(8):
+        k = int(g.integers(1, min(3, n) + 1))
+        sites = sorted(g.choice(n, size=k, replace=False) + 1)
+        if t == 0:
+            sites = [1]
+        if t == 1:
+            sites = [n]
+        facs = [sp.random(shape[s - 1], shape[s - 1], density=0.7, random_state=int(g.integers(1 << 30)),
+                          format="csc") * (1 - 0.4j) for s in sites]
+        terms.append((complex(g.standard_normal(), g.standard_normal()), tuple(int(s) for s in sites), facs))
+    psi = g.standard_normal(D) + 1j * g.standard_normal(D)
+    r0 = g.standard_normal(D) + 1j * g.standard_normal(D)
+    H = LazySumKet(shape, terms)
+    for al, be in ((1.0, 0.0), (0.3 - 0.7j, 0.2 + 0.1j)):
+        a = H.mul(r0.copy(), psi, al, be)
+        b = H.mul_flat(r0.copy(), psi, al, be)
+        assert np.array_equal(a, b)
+    ops = [O.OLazyTensor(shape, shape, t[1], tuple(O.OSparse.from_scipy(f) for f in t[2]), 1.0) for t in terms]
+    dense = sum(t[0] * O.dense_of(op) for t, op in zip(terms, ops))
+    assert O.rel_frobenius(b, al * dense @ psi + be * r0) < 1e-12
+    r = np.full(D, np.nan + 0j)
+    LazySumKet(shape, []).mul_flat(r, psi, 1, 0)
+    assert np.array_equal(r, np.zeros(D))
+
+
+def test_c_flat_engine_on_a_tfim_chain_is_bit_equal_and_fast():
+    """A 16-site transverse-field Ising chain (the batch workload's term
+    structure): flat equals recursive bit for bit, at a fraction of the time."""
+    import time
+    from cref import LazySumKet
+    from qobhip import workloads as W
+    spec = W.tfim_chain(16, 3)
+    H = LazySumKet(spec.basis.shape, spec.terms)
+    psi = W.normalized_ket(W.rng(4), 2 ** 16)
+    t0 = time.perf_counter()
+    a = H.mul(np.zeros(2 ** 16, complex), psi)
+    t1 = time.perf_counter()
+    b = H.mul_flat(np.zeros(2 ** 16, complex), psi)
+    t2 = time.perf_counter()
+    assert np.array_equal(a, b)
+    assert (t2 - t1) < (t1 - t0)
+
+
 @pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.3 - 0.7j, 0.2 + 0.1j), (0.5, 1.0)])
 def test_c_sparse_products_match_scipy(alpha, beta):
     """oracle/cpu_ref.c's sparse products (the CPU baseline of the sparse
