@@ -199,10 +199,32 @@ qob_status qob_mul_op_right(qob_ctx ctx, qob_dtype dt, const void *B,
                             int64_t rows, int64_t cols_b, qob_op a,
                             void *R, int64_t cols_r,
                             qob_c64 alpha, qob_c64 beta, qob_stream s);
+/* The same four products with the state and the result of different
+ * element types (dt_b: state, dt_r: result), as the reference allows
+ * (a ComplexF32 Ket into a ComplexF64 result, test/test_operators_lazytensor.jl:
+ * 239-253; the element type is promoted, src/operators_lazytensor.jl:32-33):
+ * the product runs in ComplexF64 and a ComplexF32 result is rounded once at
+ * the end.  Equal element types behave exactly as the entry points above. */
+qob_status qob_mul_ket_x(qob_ctx ctx, qob_dtype dt_b, qob_op a, const void *b, int64_t nb,
+                         qob_dtype dt_r, void *result, int64_t nr,
+                         qob_c64 alpha, qob_c64 beta, qob_stream s);
+qob_status qob_mul_bra_x(qob_ctx ctx, qob_dtype dt_a, const void *a, int64_t na, qob_op b,
+                         qob_dtype dt_r, void *result, int64_t nr,
+                         qob_c64 alpha, qob_c64 beta, qob_stream s);
+qob_status qob_mul_op_left_x(qob_ctx ctx, qob_dtype dt_b, qob_op a,
+                             const void *B, int64_t rows_b, int64_t cols,
+                             qob_dtype dt_r, void *R, int64_t rows_r,
+                             qob_c64 alpha, qob_c64 beta, qob_stream s);
+qob_status qob_mul_op_right_x(qob_ctx ctx, qob_dtype dt_b, const void *B,
+                              int64_t rows, int64_t cols_b, qob_op a,
+                              qob_dtype dt_r, void *R, int64_t cols_r,
+                              qob_c64 alpha, qob_c64 beta, qob_stream s);
 /* Dense x dense on raw device matrices: C = alpha * op(A) * B + beta * C with
  * op(A) = A (transa = 0) or transpose(A) (transa = 1, the Bra path of
  * src/operators_dense.jl:396).  mul!(::Operator, ::Operator, ::Operator)
- * when both operands are device matrices (src/operators_dense.jl:394). */
+ * when both operands are device matrices (src/operators_dense.jl:394), and
+ * with n = 1 the Ket and Bra forms of a device DenseOperator
+ * (src/operators_dense.jl:395-396: B is the state, C the result). */
 qob_status qob_gemm(qob_ctx ctx, qob_dtype dt, int transa, int64_t m, int64_t n,
                     int64_t k, qob_c64 alpha, const void *A, const void *B,
                     qob_c64 beta, void *C, qob_stream s);
@@ -244,7 +266,7 @@ qob_status qob_graph_destroy(qob_graph g);
  * the reference has no device launch shapes).  Names: "sell_variant",
  * "sell_cj", "sell_jchunk", "sell_unr", "csc_rb", "csc_cc", "csc_remap",
  * "csc_rows", "csc_order", "qubit_tbits", "qubit_nth", "qubit_grid",
- * "qubit_remap", "qubit_split", "qubit_enable", "modek_nth", "fused_gpw",
+ * "qubit_remap", "qubit_split", "qubit_enable", "mode_kernel", "fused_gpw",
  * "fused_lt", "fused_tiles", "fused_resident", "nt_store", "copy_variant",
  * "sell_align", "sell_closed", "sell_cjchunk", "sell_cremap", "csc_groups",
  * "csc_gkernel", "csc_gremap", "csc_waves", "csc_sweep", "ls_materialize".

@@ -11,8 +11,6 @@
 // strides, temporaries) is done once per operator and cached on its handle.
 #include "qob_internal.h"
 
-#include <rocblas/rocblas.h>
-
 #include <algorithm>
 #include <array>
 #include <complex>
@@ -34,7 +32,7 @@ namespace qob {
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
-extern int g_modek_nth, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles, g_fused_resident;
+extern int g_mode_kernel, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles, g_fused_resident;
 extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel, g_csc_gremap, g_sell_cremap;
 
 // ------------------------------------------------------------------ errors
@@ -784,9 +782,14 @@ struct qob_op_s {
 namespace qob {
 
 // ------------------------------------------------------------- workspace
-// slots: chain temporaries, reductions, scalars, expect temporary, and two
+// slots: chain temporaries, reductions, scalars, expect temporary, the
+// split partial sums of a dense matrix-vector product, the widened state and
+// result of a call whose state and result element types differ, and two
 // LazyProduct temporaries per nesting depth (up to 4)
-enum { WS_TMP1 = 0, WS_TMP2 = 1, WS_RED = 2, WS_SCAL = 3, WS_EXP = 4, WS_LP0 = 5, WS_NSLOT = 13 };
+enum {
+    WS_TMP1 = 0, WS_TMP2 = 1, WS_RED = 2, WS_SCAL = 3, WS_EXP = 4, WS_GEMV = 5, WS_MIXB = 6, WS_MIXR = 7,
+    WS_LP0 = 8, WS_NSLOT = 16
+};
 constexpr int LP_MAX_DEPTH = 4;
 
 struct Workspace {
@@ -801,8 +804,6 @@ struct qob_ctx_s {
     qob_mode mode = QOB_MODE_FUSED;
     std::mutex mu;
     std::unordered_map<hipStream_t, std::unique_ptr<qob::Workspace>> ws;
-    rocblas_handle blas = nullptr;
-    std::mutex blas_mu;
 };
 
 namespace qob {
@@ -1405,59 +1406,6 @@ static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s
     return QOB_OK;
 }
 
-// ------------------------------------------------------------------ rocBLAS
-static qob_status blas_handle(qob_ctx ctx, rocblas_handle &h) {
-    if (!ctx->blas) {
-        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-            return fail(QOB_EHIP, "rocblas_create_handle failed");
-    }
-    h = ctx->blas;
-    return QOB_OK;
-}
-
-// C (m x n) = alpha * op(A) * op(B) + beta * C, column-major, contiguous
-// leading dimensions lda/ldb/ldc, optional strided batch.
-static qob_status blas_gemm(qob_ctx ctx, hipStream_t s, qob_dtype dt, char ta, char tb, int64_t m, int64_t n,
-                            int64_t k, cd alpha, const void *A, int64_t lda, int64_t sa, const void *B,
-                            int64_t ldb, int64_t sb, cd beta, void *C, int64_t ldc, int64_t sc, int64_t batch) {
-    if (m == 0 || n == 0) return QOB_OK;
-    std::lock_guard<std::mutex> g(ctx->blas_mu);
-    rocblas_handle h;
-    QOB_TRY(blas_handle(ctx, h));
-    rocblas_set_stream(h, s);
-    auto op = [](char c) {
-        return c == 'N' ? rocblas_operation_none
-                        : (c == 'T' ? rocblas_operation_transpose : rocblas_operation_conjugate_transpose);
-    };
-    rocblas_status st;
-    if (k == 0) {
-        // no product: C = beta * C
-        if (is_zero(beta)) {
-            for (int64_t b = 0; b < batch; ++b)
-                if (hipMemsetAsync((char *)C + b * sc * esize(dt), 0, (size_t)(ldc * n) * esize(dt), s) != hipSuccess)
-                    return fail(QOB_EHIP, "hipMemsetAsync failed");
-            return QOB_OK;
-        }
-        for (int64_t b = 0; b < batch; ++b)
-            if (launch_scale(dt, ldc * n, d2(beta), (char *)C + b * sc * esize(dt), s) != hipSuccess)
-                return fail(QOB_EHIP, "scale launch failed");
-        return QOB_OK;
-    }
-    if (dt == QOB_C64) {
-        rocblas_double_complex a(alpha.real(), alpha.imag()), b(beta.real(), beta.imag());
-        st = rocblas_zgemm_strided_batched(h, op(ta), op(tb), m, n, k, &a, (const rocblas_double_complex *)A, lda, sa,
-                                           (const rocblas_double_complex *)B, ldb, sb, &b,
-                                           (rocblas_double_complex *)C, ldc, sc, batch);
-    } else {
-        rocblas_float_complex a((float)alpha.real(), (float)alpha.imag()), b((float)beta.real(), (float)beta.imag());
-        st = rocblas_cgemm_strided_batched(h, op(ta), op(tb), m, n, k, &a, (const rocblas_float_complex *)A, lda, sa,
-                                           (const rocblas_float_complex *)B, ldb, sb, &b,
-                                           (rocblas_float_complex *)C, ldc, sc, batch);
-    }
-    if (st != rocblas_status_success) return fail(QOB_EHIP, "rocBLAS gemm failed (%d)", (int)st);
-    return QOB_OK;
-}
-
 // ------------------------------------------------------------ apply engine
 // A "view" of a dense state for one application: data of size lead x D x
 // trail where the operator acts on the middle index D (decomposed by the
@@ -1524,23 +1472,25 @@ static qob_status apply_sparse(const Apply &ap, OpNode &op, const void *in, void
     return QOB_OK;
 }
 
-// DenseOperator (src/operators_dense.jl:394-396, BLAS).  The device copy of
-// the node's own matrix (M x N) is used with BLAS transposition flags.
+// DenseOperator (src/operators_dense.jl:394-396, BLAS gemm / gemv there):
+// a mode product over the whole view with the variant of the matrix the
+// application needs (op_v, or its transpose for a Bra or a right
+// application) as the factor.
+static qob_status run_mode(const Apply &ap, const void *F, int64_t d_out, int64_t d_in, const void *in, void *out,
+                           int64_t n1, int64_t n3, cd alpha, cd beta) {
+    void *ws = nullptr;
+    const size_t wb = mode_workspace_bytes(d_out, d_in, n1, n3);
+    if (wb) QOB_TRY(ws_get(ap.ctx, ap.s, WS_GEMV, wb, ws));
+    hipError_t e = launch_mode(ap.dt, F, d_out, d_in, in, out, n1, n3, d2(alpha), d2(beta), ap.s, ws);
+    if (e != hipSuccess) return fail(QOB_EHIP, "dense mode-product launch: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
 static qob_status apply_dense(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
-    const void *A;
-    QOB_TRY(dense_dev(*op.de, op.var, ap.dt, A));
-    const int64_t M = op.m, N = op.n;
-    if (ap.lead == 1) {
-        if (!ap.trans)  // out (M x trail) = A in
-            return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'N', M, ap.trail, N, alpha, A, M, 0, in, N, 0, beta, out, M, 0, 1);
-        // Bra: out (N x trail) = A^T in  (src/operators_dense.jl:396)
-        return blas_gemm(ap.ctx, ap.s, ap.dt, 'T', 'N', N, ap.trail, M, alpha, A, M, 0, in, M, 0, beta, out, N, 0, 1);
-    }
-    if (ap.trans)  // rho right: out (lead x N) = in (lead x M) A
-        return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'N', ap.lead, N, M, alpha, in, ap.lead, 0, A, M, 0, beta, out,
-                         ap.lead, 0, 1);
-    return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'T', ap.lead, M, N, alpha, in, ap.lead, 0, A, M, 0, beta, out, ap.lead,
-                     0, 1);
+    const void *F;
+    QOB_TRY(dense_dev(*op.de, op.var ^ (ap.trans ? V_T : V_N), ap.dt, F));
+    return run_mode(ap, F, op_out_dim(op, ap.trans), op_in_dim(op, ap.trans), in, out, ap.lead, ap.trail, alpha,
+                    beta);
 }
 
 // One step of the tensor-product chain: mode product on site k of the
@@ -1557,21 +1507,10 @@ static qob_status chain_step(const Apply &ap, OpNode &owner, const std::string &
     int64_t n1 = ap.lead * prod(shape_in, 0, c.site);
     int64_t n3 = prod(shape_in, c.site + 1, N) * ap.trail;
     OpNode &f = *c.f;
-    if (f.kind == K_DENSE && c.d_in <= 64) {
-        const void *A;
-        QOB_TRY(dense_dev(*f.de, f.var ^ (ap.trans ? V_T : V_N), ap.dt, A));
-        hipError_t e = launch_modek(ap.dt, A, c.d_out, c.d_in, in, out, n1, n3, d2(alpha), d2(beta), ap.s);
-        if (e != hipSuccess) return fail(QOB_EHIP, "mode-k launch: %s", hipGetErrorString(e));
-        return QOB_OK;
-    }
     if (f.kind == K_DENSE) {
         const void *A;
         QOB_TRY(dense_dev(*f.de, f.var ^ (ap.trans ? V_T : V_N), ap.dt, A));  // d_out x d_in
-        if (n1 == 1)
-            return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'N', c.d_out, n3, c.d_in, alpha, A, c.d_out, 0, in, c.d_in, 0,
-                             beta, out, c.d_out, 0, 1);
-        return blas_gemm(ap.ctx, ap.s, ap.dt, 'N', 'T', n1, c.d_out, c.d_in, alpha, in, n1, n1 * c.d_in, A, c.d_out,
-                         0, beta, out, n1, n1 * c.d_out, n3);
+        return run_mode(ap, A, c.d_out, c.d_in, in, out, n1, n3, alpha, beta);
     }
     // sparse factor or isometry: one-term gather over the full view
     std::vector<int64_t> full_in, full_out;
@@ -2100,7 +2039,6 @@ qob_status qob_ctx_destroy(qob_ctx ctx) {
     if (!ctx) return QOB_OK;
     (void)hipStreamSynchronize(ctx->stream);
     ctx->ws.clear();
-    if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return QOB_OK;
@@ -2540,14 +2478,98 @@ qob_status qob_mul_op_right(qob_ctx ctx, qob_dtype dt, const void *B, int64_t ro
     return apply_op(ap, op, B, R, cdv(alpha), cdv(beta));
 }
 
+// State and result of different element types: the product runs in
+// ComplexF64 (the promoted type, src/operators_lazytensor.jl:32-33), the
+// ComplexF32 operand widened into a workspace first (exactly) and a
+// ComplexF32 result rounded once at the end.
+}  // extern "C"
+
+template <typename F>
+static qob_status run_mixed(qob_ctx ctx, qob_stream s, qob_dtype dt_b, const void *b, int64_t nb, qob_dtype dt_r,
+                            void *r, int64_t nr, qob_c64 beta, F &&call) {
+    if ((dt_b != QOB_C64 && dt_b != QOB_C32) || (dt_r != QOB_C64 && dt_r != QOB_C32))
+        return fail(QOB_EARG, "bad dtype %d / %d", (int)dt_b, (int)dt_r);
+    if (dt_b == dt_r) return call(dt_r, b, r);
+    if (b == r && b) return fail(QOB_EALIAS, "output must not be aliased with input");
+    hipStream_t st = strm(ctx, s);
+    const void *bw = b;
+    void *rw = r;
+    if (dt_b == QOB_C32 && nb > 0) {
+        void *t;
+        QOB_TRY(ws_get(ctx, st, WS_MIXB, (size_t)nb * 16, t));
+        QOB_HIP(launch_convert(QOB_C32, b, QOB_C64, t, nb, st));
+        bw = t;
+    }
+    if (dt_r == QOB_C32 && nr > 0) {
+        void *t;
+        QOB_TRY(ws_get(ctx, st, WS_MIXR, (size_t)nr * 16, t));
+        if (!is_zero(cdv(beta))) QOB_HIP(launch_convert(QOB_C32, r, QOB_C64, t, nr, st));
+        rw = t;
+    }
+    QOB_TRY(call(QOB_C64, bw, rw));
+    if (dt_r == QOB_C32 && nr > 0) QOB_HIP(launch_convert(QOB_C64, rw, QOB_C32, r, nr, st));
+    return QOB_OK;
+}
+
+extern "C" {
+
+qob_status qob_mul_ket_x(qob_ctx ctx, qob_dtype dt_b, qob_op a, const void *b, int64_t nb, qob_dtype dt_r,
+                         void *result, int64_t nr, qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    return run_mixed(ctx, s, dt_b, b, nb, dt_r, result, nr, beta, [&](qob_dtype dt, const void *bb, void *rr) {
+        return qob_mul_ket(ctx, dt, a, bb, nb, rr, nr, alpha, beta, s);
+    });
+}
+
+qob_status qob_mul_bra_x(qob_ctx ctx, qob_dtype dt_a, const void *a, int64_t na, qob_op b, qob_dtype dt_r,
+                         void *result, int64_t nr, qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    return run_mixed(ctx, s, dt_a, a, na, dt_r, result, nr, beta, [&](qob_dtype dt, const void *aa, void *rr) {
+        return qob_mul_bra(ctx, dt, aa, na, b, rr, nr, alpha, beta, s);
+    });
+}
+
+qob_status qob_mul_op_left_x(qob_ctx ctx, qob_dtype dt_b, qob_op a, const void *B, int64_t rows_b, int64_t cols,
+                             qob_dtype dt_r, void *R, int64_t rows_r, qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    if (rows_b < 0 || rows_r < 0 || cols < 0) return fail(QOB_EDIM, "negative dimension");
+    return run_mixed(ctx, s, dt_b, B, rows_b * cols, dt_r, R, rows_r * cols, beta,
+                     [&](qob_dtype dt, const void *bb, void *rr) {
+                         return qob_mul_op_left(ctx, dt, a, bb, rows_b, cols, rr, rows_r, alpha, beta, s);
+                     });
+}
+
+qob_status qob_mul_op_right_x(qob_ctx ctx, qob_dtype dt_b, const void *B, int64_t rows, int64_t cols_b, qob_op a,
+                              qob_dtype dt_r, void *R, int64_t cols_r, qob_c64 alpha, qob_c64 beta, qob_stream s) {
+    if (!ctx) return fail(QOB_EARG, "null context");
+    if (rows < 0 || cols_b < 0 || cols_r < 0) return fail(QOB_EDIM, "negative dimension");
+    return run_mixed(ctx, s, dt_b, B, rows * cols_b, dt_r, R, rows * cols_r, beta,
+                     [&](qob_dtype dt, const void *bb, void *rr) {
+                         return qob_mul_op_right(ctx, dt, bb, rows, cols_b, a, rr, cols_r, alpha, beta, s);
+                     });
+}
+
 qob_status qob_gemm(qob_ctx ctx, qob_dtype dt, int transa, int64_t m, int64_t n, int64_t k, qob_c64 alpha,
                     const void *A, const void *B, qob_c64 beta, void *C, qob_stream s) {
     if (!ctx) return fail(QOB_EARG, "null context");
     if (C == A || C == B) return fail(QOB_EALIAS, "output must not be aliased with input");
     if (m < 0 || n < 0 || k < 0) return fail(QOB_EDIM, "negative dimension");
-    const int64_t lda = transa ? k : m;
-    return blas_gemm(ctx, strm(ctx, s), dt, transa ? 'T' : 'N', 'N', m, n, k, cdv(alpha), A, lda, 0, B, k, 0,
-                     cdv(beta), C, m, 0, 1);
+    hipStream_t st = strm(ctx, s);
+    hipError_t e;
+    if (k == 0) {  // no product: C = beta C (beta = 0 writes zeros)
+        e = launch_scale(dt, m * n, d2(beta), C, st);
+    } else if (n == 1 && !transa) {  // matrix-vector (src/operators_dense.jl:395)
+        void *ws = nullptr;
+        const int64_t ns = zgemv_splits(m, k);
+        if (ns > 1) QOB_TRY(ws_get(ctx, st, WS_GEMV, (size_t)(ns * m) * 16, ws));
+        e = launch_zgemv(dt, A, m, k, B, C, ws, d2(alpha), d2(beta), st);
+    } else if (n == 1) {  // transpose(A) x (the Bra form, src/operators_dense.jl:396)
+        e = launch_zgemv_t(dt, A, k, m, B, C, d2(alpha), d2(beta), st);
+    } else  // op(A) (m x k): A column-major m x k, or k x m read transposed
+        e = launch_zgemm(dt, m, n, k, A, transa ? k : 1, transa ? 1 : m, 0, B, 1, k, 0, C, 1, m, 0, 1, d2(alpha),
+                         d2(beta), st);
+    if (e != hipSuccess) return fail(QOB_EHIP, "gemm launch: %s", hipGetErrorString(e));
+    return QOB_OK;
 }
 
 // ----------------------------------------------------- elementwise / reduce
@@ -2576,7 +2598,7 @@ static int *tune_slot(const char *name) {
                {"fused_lt", &g_fused_lt},         {"qubit_enable", &g_qubit_enable},
                {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
-               {"qubit_split", &g_qubit_split},   {"modek_nth", &g_modek_nth},
+               {"qubit_split", &g_qubit_split},   {"mode_kernel", &g_mode_kernel},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
@@ -2599,8 +2621,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "sell_cj") && value != 1 && value != 2 && value != 4 && value != 8)
         return fail(QOB_EARG, "sell_cj must be 1, 2, 4 or 8");
     if (!strcmp(name, "qubit_tbits") && (value < 10 || value > 13)) return fail(QOB_EARG, "qubit_tbits must be in 10..13");
-    if (!strcmp(name, "modek_nth") && value != 64 && value != 128 && value != 256 && value != 512)
-        return fail(QOB_EARG, "modek_nth must be 64, 128, 256 or 512");
+    if (!strcmp(name, "mode_kernel") && value != 0 && value != 1)
+        return fail(QOB_EARG, "mode_kernel must be 0 (automatic) or 1 (tiled GEMM for every dense mode product)");
     if (!strcmp(name, "sell_unr") && value != 2 && value != 4 && value != 8)
         return fail(QOB_EARG, "sell_unr must be 2, 4 or 8");
     if (!strcmp(name, "csc_sweep") && value > 2) return fail(QOB_EARG, "csc_sweep must be 0, 1 or 2");

@@ -28,37 +28,14 @@
 // ComplexF64 uses FP64 MFMA; ComplexF32 uses FP32 MFMA with FP32
 // accumulation.  Columns are distributed over persistent workgroups.
 //
-// MFMA lane layouts (checked on gfx950 with a one-wave probe):
-//   A-frag  A[i][k]  lane i + 16 k          (i < 16, k < 4)
-//   B-frag  B[k][j]  lane j + 16 k
-//   f32 D   D[4 (lane/16) + r][lane % 16]   register r
-//   f64 D   D[(lane/16) + 4 r][lane % 16]   register r
-#include "qob_internal.h"
+// MFMA lane layouts: qob_mfma.h.
+#include "qob_mfma.h"
 
 #include <atomic>
 
 namespace qob {
 
 namespace {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <typename R> struct mf;
-template <> struct mf<double> {
-    using v4 = v4d;
-    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
-template <> struct mf<float> {
-    using v4 = v4f;
-    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 

@@ -203,13 +203,26 @@ hipError_t launch_gather_cols(qob_dtype dt, const GatherPlanDev &P, const void *
                               void *out, int64_t n_out, int64_t nrows, double2 alpha,
                               double2 beta, hipStream_t s);
 
-// dense mode-k (k_modek.hip)
-// out viewed (n1, d_out, n3), in viewed (n1, d_in, n3);
-// out = alpha * A x_2 in + beta * out, A is d_out x d_in column-major.
-hipError_t launch_modek(qob_dtype dt, const void *A, int64_t d_out, int64_t d_in, const void *in,
-                        void *out, int64_t n1, int64_t n3, double2 alpha, double2 beta,
-                        hipStream_t s);
-bool modek_supported(int64_t d_out, int64_t d_in);
+// dense mode products (k_mode.hip)
+// out viewed (n1, d_out, n3), in viewed (n1, d_in, n3), column-major;
+// out = alpha * F x_2 in + beta * out, F is d_out x d_in column-major.
+// ws: mode_workspace_bytes() bytes of device memory (null when that is 0).
+hipError_t launch_mode(qob_dtype dt, const void *F, int64_t d_out, int64_t d_in, const void *in, void *out,
+                       int64_t n1, int64_t n3, double2 alpha, double2 beta, hipStream_t s, void *ws);
+size_t mode_workspace_bytes(int64_t d_out, int64_t d_in, int64_t n1, int64_t n3);
+// y = alpha * F x + beta * y for an m x n column-major F (part: split
+// partial sums, zgemv_splits(m, n) * m complex doubles when that is > 1), and
+// y = alpha * A^T x + beta * y for a k x m column-major A.
+int64_t zgemv_splits(int64_t m, int64_t n);
+hipError_t launch_zgemv(qob_dtype dt, const void *F, int64_t m, int64_t n, const void *x, void *y, void *part,
+                        double2 alpha, double2 beta, hipStream_t s);
+hipError_t launch_zgemv_t(qob_dtype dt, const void *A, int64_t k, int64_t m, const void *x, void *y, double2 alpha,
+                          double2 beta, hipStream_t s);
+// C[m, n] = alpha * sum_k A[m, k] B[k, n] + beta * C[m, n] for z < batch, with
+// element strides (A: m, k, batch; B: k, n, batch; C: m, n, batch); K > 0.
+hipError_t launch_zgemm(qob_dtype dt, int64_t M, int64_t N, int64_t K, const void *A, int64_t sam, int64_t sak,
+                        int64_t sa, const void *B, int64_t sbk, int64_t sbn, int64_t sb, void *C, int64_t scm,
+                        int64_t scn, int64_t sc, int64_t batch, double2 alpha, double2 beta, hipStream_t s);
 
 // fused dense LazyTensor on a column (k_fused.hip): modes applied in order,
 // mode s acts on dimension d[s] with n1[s] = product of the dims before it;

@@ -103,6 +103,10 @@ _PROTOS = {
     "qob_mul_op_left": (_i32, [_vp, _i32, _vp, _vp, _i64, _i64, _vp, _i64, qob_c64, qob_c64, _vp]),
     "qob_mul_op_right": (_i32, [_vp, _i32, _vp, _i64, _i64, _vp, _vp, _i64, qob_c64, qob_c64, _vp]),
     "qob_gemm": (_i32, [_vp, _i32, _i32, _i64, _i64, _i64, qob_c64, _vp, _vp, qob_c64, _vp, _vp]),
+    "qob_mul_ket_x": (_i32, [_vp, _i32, _vp, _vp, _i64, _i32, _vp, _i64, qob_c64, qob_c64, _vp]),
+    "qob_mul_bra_x": (_i32, [_vp, _i32, _vp, _i64, _vp, _i32, _vp, _i64, qob_c64, qob_c64, _vp]),
+    "qob_mul_op_left_x": (_i32, [_vp, _i32, _vp, _vp, _i64, _i64, _i32, _vp, _i64, qob_c64, qob_c64, _vp]),
+    "qob_mul_op_right_x": (_i32, [_vp, _i32, _vp, _i64, _i64, _vp, _i32, _vp, _i64, qob_c64, qob_c64, _vp]),
     "qob_scale": (_i32, [_vp, _i32, _i64, qob_c64, _vp, _vp]),
     "qob_axpy": (_i32, [_vp, _i32, _i64, qob_c64, _vp, _vp, _vp]),
     "qob_copy": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp]),

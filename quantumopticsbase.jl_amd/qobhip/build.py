@@ -1,7 +1,8 @@
 """Build libqob_hip.so for gfx950 in-tree.
 
 Compiles every ``csrc/*.hip`` / ``csrc/*.cpp`` with hipcc into
-``_build/`` and links ``lib/libqob_hip.so`` (rocBLAS for plain dense GEMMs).
+``_build/`` and links ``lib/libqob_hip.so`` (HIP runtime only: every product
+runs on the library's own kernels).
 Used by ``__graft_entry__.build()``; also runnable as
 ``python quantumopticsbase.jl_amd/qobhip/build.py``.
 """
@@ -31,7 +32,7 @@ CFLAGS = [
     # keep hipcc from forming packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-LDFLAGS = ["-shared", "-fPIC", "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+LDFLAGS = ["-shared", "-fPIC"]
 
 
 def sources() -> list[Path]:

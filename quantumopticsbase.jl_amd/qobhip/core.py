@@ -634,47 +634,62 @@ def mul_(result, a, b, alpha=1.0, beta=0.0):
         if not (isinstance(b, Ket) and isinstance(a, AbstractOperator)):
             raise MethodError(f"no method mul!(::Ket, ::{type(a).__name__}, ::{type(b).__name__})")
         _bases(a.basis_r, b.basis, a.basis_l, result.basis)
-        dt = _check_same_dtype(result, b)
         if result.data.ptr.value == b.data.ptr.value:
             raise ArgumentError("output must not be aliased with input")
         if isinstance(a, Operator):
-            _gemm_dense(dt, 0, len(a.basis_l), 1, len(a.basis_r), alpha, a, b.data, beta, result.data)
+            _gemm_dense(0, len(a.basis_l), 1, len(a.basis_r), alpha, a, b.data, beta, result.data)
             return result
-        call("qob_mul_ket", ctx.h, dt, a._handle(), b.data.ptr, b.data.n, result.data.ptr, result.data.n, al, be,
-             None)
+        dtb, dtr = _dt(b.dtype), _dt(result.dtype)
+        if dtb == dtr:
+            call("qob_mul_ket", ctx.h, dtr, a._handle(), b.data.ptr, b.data.n, result.data.ptr, result.data.n, al,
+                 be, None)
+        else:
+            call("qob_mul_ket_x", ctx.h, dtb, a._handle(), b.data.ptr, b.data.n, dtr, result.data.ptr,
+                 result.data.n, al, be, None)
         return result
     if isinstance(result, Bra):
         if not (isinstance(a, Bra) and isinstance(b, AbstractOperator)):
             raise MethodError(f"no method mul!(::Bra, ::{type(a).__name__}, ::{type(b).__name__})")
         _bases(a.basis, b.basis_l, result.basis, b.basis_r)
-        dt = _check_same_dtype(result, a)
         if isinstance(b, Operator):
-            _gemm_dense(dt, 1, len(b.basis_r), 1, len(b.basis_l), alpha, b, a.data, beta, result.data)
+            _gemm_dense(1, len(b.basis_r), 1, len(b.basis_l), alpha, b, a.data, beta, result.data)
             return result
-        call("qob_mul_bra", ctx.h, dt, a.data.ptr, a.data.n, b._handle(), result.data.ptr, result.data.n, al, be,
-             None)
+        dta, dtr = _dt(a.dtype), _dt(result.dtype)
+        if dta == dtr:
+            call("qob_mul_bra", ctx.h, dtr, a.data.ptr, a.data.n, b._handle(), result.data.ptr, result.data.n, al,
+                 be, None)
+        else:
+            call("qob_mul_bra_x", ctx.h, dta, a.data.ptr, a.data.n, b._handle(), dtr, result.data.ptr,
+                 result.data.n, al, be, None)
         return result
     if isinstance(result, Operator):
         if isinstance(b, Operator) and isinstance(a, AbstractOperator) and not (
                 isinstance(a, Operator) and a is result):
             if isinstance(a, Operator):
                 _bases(a.basis_r, b.basis_l, a.basis_l, result.basis_l)
-                dt = _check_same_dtype(result, a, b)
-                _gemm_dense(dt, 0, len(a.basis_l), len(b.basis_r), len(a.basis_r), alpha, a, b.data, beta,
+                _gemm_dense(0, len(a.basis_l), len(b.basis_r), len(a.basis_r), alpha, a, b.data, beta,
                             result.data)
                 return result
             _bases(a.basis_r, b.basis_l, a.basis_l, result.basis_l)
             _bases(b.basis_r, result.basis_r)
-            dt = _check_same_dtype(result, b)
-            call("qob_mul_op_left", ctx.h, dt, a._handle(), b.data.ptr, len(b.basis_l), len(b.basis_r),
-                 result.data.ptr, len(result.basis_l), al, be, None)
+            dtb, dtr = _dt(b.dtype), _dt(result.dtype)
+            if dtb == dtr:
+                call("qob_mul_op_left", ctx.h, dtr, a._handle(), b.data.ptr, len(b.basis_l), len(b.basis_r),
+                     result.data.ptr, len(result.basis_l), al, be, None)
+            else:
+                call("qob_mul_op_left_x", ctx.h, dtb, a._handle(), b.data.ptr, len(b.basis_l), len(b.basis_r),
+                     dtr, result.data.ptr, len(result.basis_l), al, be, None)
             return result
         if isinstance(a, Operator) and isinstance(b, AbstractOperator):
             _bases(a.basis_r, b.basis_l, a.basis_l, result.basis_l)
             _bases(b.basis_r, result.basis_r)
-            dt = _check_same_dtype(result, a)
-            call("qob_mul_op_right", ctx.h, dt, a.data.ptr, len(a.basis_l), len(a.basis_r), b._handle(),
-                 result.data.ptr, len(result.basis_r), al, be, None)
+            dta, dtr = _dt(a.dtype), _dt(result.dtype)
+            if dta == dtr:
+                call("qob_mul_op_right", ctx.h, dtr, a.data.ptr, len(a.basis_l), len(a.basis_r), b._handle(),
+                     result.data.ptr, len(result.basis_r), al, be, None)
+            else:
+                call("qob_mul_op_right_x", ctx.h, dta, a.data.ptr, len(a.basis_l), len(a.basis_r), b._handle(),
+                     dtr, result.data.ptr, len(result.basis_r), al, be, None)
             return result
     raise MethodError(f"no method mul!(::{type(result).__name__}, ::{type(a).__name__}, ::{type(b).__name__})")
 
@@ -692,14 +707,36 @@ def _generic(b: Basis) -> bool:
     return isinstance(b, GenericBasis)
 
 
-def _gemm_dense(dt, transa, m, n, k, alpha, A: Operator, B: DeviceArray, beta, Cd: DeviceArray):
-    if _dt(A.dtype) != dt:
-        raise ArgumentError("dense operator and state must share an element type")
+def _widened(d: DeviceArray, copy_in: bool = True) -> DeviceArray:
+    """A ComplexF64 copy of a ComplexF32 device array (exact); ComplexF64
+    arrays are returned as they are."""
+    if d.dtype == np.complex128:
+        return d
+    w = DeviceArray(d.n, np.complex128, d.shape)
+    if copy_in and d.n:
+        call("qob_convert", context().h, QOB_C32, d.ptr, QOB_C64, w.ptr, d.n, None)
+    return w
+
+
+def _gemm_dense(transa, m, n, k, alpha, A: Operator, B: DeviceArray, beta, Cd: DeviceArray):
+    """C = alpha op(A) B + beta C for a device DenseOperator A
+    (src/operators_dense.jl:394-396).  When the element types of A, B and C
+    differ the product runs in ComplexF64 (the promoted type) and a
+    ComplexF32 C is rounded once at the end."""
     if Cd.ptr.value in (A.data.ptr.value, B.ptr.value):
         raise ArgumentError("output must not be aliased with input")
     if Cd.n != m * n or B.n != k * n:
         raise DimensionMismatch(f"dense mul!: A {m}x{k}, B {B.n} elements, C {Cd.n} elements")
-    call("qob_gemm", context().h, dt, transa, m, n, k, c64(alpha), A.data.ptr, B.ptr, c64(beta), Cd.ptr, None)
+    h = context().h
+    if A.data.dtype == B.dtype == Cd.dtype:
+        call("qob_gemm", h, _dt(Cd.dtype), transa, m, n, k, c64(alpha), A.data.ptr, B.ptr, c64(beta), Cd.ptr, None)
+        return
+    Aw, Bw = _widened(A.data), _widened(B)
+    Cw = _widened(Cd, copy_in=complex(beta) != 0)
+    call("qob_gemm", h, QOB_C64, transa, m, n, k, c64(alpha), Aw.ptr, Bw.ptr, c64(beta), Cw.ptr, None)
+    if Cw is not Cd:
+        call("qob_convert", h, QOB_C64, Cw.ptr, QOB_C32, Cd.ptr, Cd.n, None)
+    call("qob_stream_synchronize", h, None)  # the temporaries are freed on return
 
 
 def _alloc_mul(a, b):

@@ -3,10 +3,11 @@
 No reference counterpart.  The kernels' launch shapes are runtime tunables
 of libqob_hip.so (qob_tune_set / qob_tune_get):
 
-  mode-k products   modek_nth (fibres per workgroup), fused_gpw (fibre
-                    groups in flight per wave in the fused MFMA kernel;
-                    0 = by precision and factor width),
-                    fused_resident (factors kept in LDS for the launch)
+  mode products     fused_gpw (fibre groups in flight per wave in the
+                    fused MFMA kernel; 0 = by precision and factor width),
+                    fused_resident (factors kept in LDS for the launch);
+                    mode_kernel (1 = tiled GEMM for every dense mode product,
+                    a test and timing switch, not tuned)
   qubit sums        qubit_tbits (tile = 2^tbits states), qubit_nth,
                     qubit_grid (persistent workgroups; 0 = 3 per CU for ComplexF64, 4 for ComplexF32),
                     qubit_remap (XCD-contiguous tiles), qubit_split
@@ -42,7 +43,6 @@ from . import _lib
 from ._lib import call
 
 SPACES: Dict[str, Dict[str, List[int]]] = {
-    "modek": {"modek_nth": [64, 128, 256, 512]},
     "fused": {"fused_gpw": [0, 1, 2, 4], "fused_resident": [1, 0]},
     "qubit": {"qubit_tbits": [10, 11, 12, 13], "qubit_nth": [256, 512, 1024], "qubit_grid": [0, 768, 1024, 2048],
               "qubit_remap": [0, 1], "qubit_split": [0, 1]},
@@ -222,7 +222,7 @@ def _workloads(families):
     from . import workloads as W
     from .batch import Batch
     runs = {}
-    if "modek" in families or "fused" in families:
+    if "fused" in families:
         N = 8192
         LT = W.dense_lazytensor((16, 32, 16), 1)
         b = Q.GenericBasis(N)
@@ -230,8 +230,7 @@ def _workloads(families):
         # any contents: only the launch shapes are timed
         call("qob_scale", Q.context().h, Q.QOB_C64, N * N, _lib.c64(0.0), rho_d.data.ptr, None)
         out_d = Q.Operator(b, b, Q.DeviceArray(N * N))
-        # right apply runs the mode-k chain; left apply the fused MFMA kernel
-        runs["modek"] = lambda: (lambda: Q.mul_(out_d, rho_d, LT, 1.0, 0.0))
+        # the left apply runs the fused MFMA kernel
         runs["fused"] = lambda: (lambda: Q.mul_(out_d, LT, rho_d, 1.0, 0.0))
     if "qubit" in families:
         Ls = W.tfim_batch_lengths(256, 8, 20, seed=1)
@@ -260,7 +259,7 @@ def _workloads(families):
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="tune libqob_hip launch shapes on the benchmark workloads")
-    ap.add_argument("--families", default="modek,fused,qubit,sparse_left,sparse_right")
+    ap.add_argument("--families", default="fused,qubit,sparse_left,sparse_right")
     ap.add_argument("--out", default=str(PACKAGED))
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args(argv)

@@ -33,17 +33,17 @@ def test_coordinate_search_interaction_reaches_coordinate_optimum():
 
 
 def test_save_load_roundtrip(tmp_path):
-    names = ["qubit_tbits", "qubit_nth", "modek_nth", "fused_gpw"]
+    names = ["qubit_tbits", "qubit_nth", "mode_kernel", "fused_gpw"]
     saved = T.snapshot(names)
     try:
         res = [{"family": "qubit", "best": {"qubit_tbits": 12, "qubit_nth": 256}, "best_ms": 1.0, "trials": []},
-               {"family": "modek", "best": {"modek_nth": 128}, "best_ms": 2.0, "trials": []}]
+               {"family": "mode", "best": {"mode_kernel": 1}, "best_ms": 2.0, "trials": []}]
         path = tmp_path / "tuned.json"
         doc = T.save(path, res)
-        assert json.loads(path.read_text())["tunables"] == {"qubit_tbits": 12, "qubit_nth": 256, "modek_nth": 128}
+        assert json.loads(path.read_text())["tunables"] == {"qubit_tbits": 12, "qubit_nth": 256, "mode_kernel": 1}
         T.apply(saved)
         T.load(path)
-        assert T.get("qubit_tbits") == 12 and T.get("qubit_nth") == 256 and T.get("modek_nth") == 128
+        assert T.get("qubit_tbits") == 12 and T.get("qubit_nth") == 256 and T.get("mode_kernel") == 1
         # a file tuned on another device model is ignored
         doc["device"] = "some other GPU"
         doc["tunables"] = {"qubit_tbits": 13}
@@ -55,7 +55,7 @@ def test_save_load_roundtrip(tmp_path):
 
 def test_invalid_tunable_values_are_rejected():
     from qobhip import ArgumentError
-    for name, bad in [("modek_nth", 100), ("fused_gpw", 3), ("qubit_split", 2), ("qubit_tbits", 9)]:
+    for name, bad in [("mode_kernel", 2), ("fused_gpw", 3), ("qubit_split", 2), ("qubit_tbits", 9)]:
         with pytest.raises(ArgumentError):
             T.set_(name, bad)
 
