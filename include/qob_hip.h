@@ -266,7 +266,7 @@ qob_status qob_graph_destroy(qob_graph g);
  * the reference has no device launch shapes).  Names: "sell_variant",
  * "sell_cj", "sell_jchunk", "sell_unr", "csc_rb", "csc_cc", "csc_remap",
  * "csc_rows", "csc_order", "qubit_tbits", "qubit_nth", "qubit_grid",
- * "qubit_remap", "qubit_split", "qubit_enable", "mode_kernel", "fused_gpw",
+ * "qubit_remap", "qubit_split", "qubit_enable", "mode_kernel", "fused_reg", "fused_gpw",
  * "fused_lt", "fused_tiles", "fused_resident", "nt_store", "copy_variant",
  * "sell_align", "sell_closed", "sell_cjchunk", "sell_cremap", "csc_groups",
  * "csc_gkernel", "csc_gremap", "csc_waves", "csc_sweep", "ls_materialize".
@@ -341,6 +341,11 @@ qob_status qob_expect_op(qob_ctx ctx, qob_dtype dt, qob_op a, const void *rho,
  * sum or an all-sparse lazy operator. */
 qob_status qob_variance_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *psi,
                             int64_t n, qob_c64 *out, qob_stream s);
+/* variance(op, rho) = tr(op op rho) - tr(op rho)^2 with op * rho formed once
+ * into the library's workspace (the operator form of src/operators.jl:132-143);
+ * FP64 result for both element types. */
+qob_status qob_variance_op(qob_ctx ctx, qob_dtype dt, qob_op a, const void *rho,
+                           int64_t n, qob_c64 *out, qob_stream s);
 qob_status qob_dot_d(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x,
                      const void *y, void *dout, qob_stream s);
 qob_status qob_norm_d(qob_ctx ctx, qob_dtype dt, int64_t n, const void *x,

@@ -17,12 +17,14 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 using cd = std::complex<double>;
@@ -33,6 +35,7 @@ extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_r
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
 extern int g_mode_kernel, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles, g_fused_resident;
+extern int g_fused_reg;
 extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel, g_csc_gremap, g_sell_cremap;
 
 // ------------------------------------------------------------------ errors
@@ -107,6 +110,14 @@ struct DevBuf {
         QOB_TRY(alloc(b));
         if (b) QOB_HIP(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
         return QOB_OK;
+    }
+    // hand the allocation over (freed when the last owner lets go); empty after
+    std::shared_ptr<void> release() {
+        std::shared_ptr<void> r;
+        if (p) r = std::shared_ptr<void>(p, [](void *q) { (void)hipFree(q); });
+        p = nullptr;
+        bytes = 0;
+        return r;
     }
 };
 
@@ -777,6 +788,7 @@ static bool lt_pure_sparse(const OpNode &o) {
 
 struct qob_op_s {
     std::shared_ptr<qob::OpNode> node;
+    qob_ctx ctx = nullptr;  // the context that created the handle
 };
 
 namespace qob {
@@ -784,16 +796,24 @@ namespace qob {
 // ------------------------------------------------------------- workspace
 // slots: chain temporaries, reductions, scalars, expect temporary, the
 // split partial sums of a dense matrix-vector product, the widened state and
-// result of a call whose state and result element types differ, and two
-// LazyProduct temporaries per nesting depth (up to 4)
+// result of a call whose state and result element types differ, op * rho of
+// a variance, and two LazyProduct temporaries per nesting depth (up to 4)
 enum {
     WS_TMP1 = 0, WS_TMP2 = 1, WS_RED = 2, WS_SCAL = 3, WS_EXP = 4, WS_GEMV = 5, WS_MIXB = 6, WS_MIXR = 7,
-    WS_LP0 = 8, WS_NSLOT = 16
+    WS_VAR = 8, WS_LP0 = 9, WS_NSLOT = 17
 };
 constexpr int LP_MAX_DEPTH = 4;
 
 struct Workspace {
     DevBuf slot[WS_NSLOT];
+};
+
+// Device memory that queued work, or a live captured graph, may still read
+// when the library replaces it: kept until it is safe to free.
+struct Retired {
+    std::vector<std::shared_ptr<void>> keep;
+    std::vector<hipEvent_t> evs;  // recorded on streams that may read it
+    bool need_sync = false;       // wait for a device synchronisation as well
 };
 
 }  // namespace qob
@@ -804,10 +824,85 @@ struct qob_ctx_s {
     qob_mode mode = QOB_MODE_FUSED;
     std::mutex mu;
     std::unordered_map<hipStream_t, std::unique_ptr<qob::Workspace>> ws;
+    // captured graphs alive and memory retired while they (or queued work) may use it
+    std::mutex rmu;
+    int live_graphs = 0;
+    std::deque<qob::Retired> retired;
+    // streams created by qob_stream_create (alive until qob_stream_destroy),
+    // and whether work was ever issued on a stream the library did not create
+    std::unordered_set<hipStream_t> lib_streams;
+    bool foreign_streams = false;
 };
 
 namespace qob {
 
+
+// Free retired memory whose events have completed, in retirement order, as
+// long as no captured graph of the context is alive (a graph may replay any
+// of it).  synced: the device was just synchronised.
+static void reap_locked(qob_ctx ctx, bool synced) {
+    if (ctx->live_graphs > 0) return;
+    while (!ctx->retired.empty()) {
+        Retired &R = ctx->retired.front();
+        bool done = synced || !R.need_sync;
+        for (size_t i = 0; done && i < R.evs.size(); ++i)
+            if (!synced && hipEventQuery(R.evs[i]) == hipErrorNotReady) done = false;
+        if (!done) break;
+        for (hipEvent_t e : R.evs) (void)hipEventDestroy(e);
+        ctx->retired.pop_front();
+    }
+}
+
+static bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// Keep `keep` until the work queued so far that may read it has finished and
+// no captured graph is alive, instead of freeing it now.  only != null: only
+// work on that stream may read it (a workspace of that stream).  Otherwise
+// events go on the context's stream and on the streams the library created;
+// if work was ever issued on other streams (whose lifetime the library does
+// not know), the memory waits for a device synchronisation instead, which
+// runs at the latest every 8th such retirement.
+static void retire(qob_ctx ctx, std::vector<std::shared_ptr<void>> keep, hipStream_t only = nullptr) {
+    if (keep.empty()) return;
+    std::vector<hipStream_t> streams;
+    bool foreign;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        foreign = ctx->foreign_streams && !only;
+        if (only) {
+            streams.push_back(only);
+        } else {
+            streams.push_back(ctx->stream);
+            for (hipStream_t x : ctx->lib_streams) streams.push_back(x);
+        }
+    }
+    Retired R;
+    R.keep = std::move(keep);
+    R.need_sync = foreign;
+    for (hipStream_t x : streams) {
+        hipEvent_t e = nullptr;
+        if (capturing(x) || hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            R.need_sync = true;
+            continue;
+        }
+        if (hipEventRecord(e, x) != hipSuccess) {
+            (void)hipEventDestroy(e);
+            R.need_sync = true;
+            continue;
+        }
+        R.evs.push_back(e);
+    }
+    std::lock_guard<std::mutex> g(ctx->rmu);
+    ctx->retired.push_back(std::move(R));
+    size_t waiting = 0;
+    for (const Retired &x : ctx->retired) waiting += x.need_sync;
+    bool synced = false;
+    if (waiting >= 8 && ctx->live_graphs == 0 && !capturing(ctx->stream)) synced = hipDeviceSynchronize() == hipSuccess;
+    reap_locked(ctx, synced);
+}
 
 static qob_status ws_get(qob_ctx ctx, hipStream_t s, int slot, size_t bytes, void *&p) {
     Workspace *w;
@@ -818,15 +913,29 @@ static qob_status ws_get(qob_ctx ctx, hipStream_t s, int slot, size_t bytes, voi
         w = u.get();
     }
     if (w->slot[slot].bytes < bytes) {
-        // growing frees the old buffer; queued work on s may still read it
-        QOB_HIP(hipStreamSynchronize(s));
+        if (capturing(s))
+            return fail(QOB_EUNSUPPORTED,
+                        "a workspace would grow (to %zu bytes) during stream capture: run the call once before "
+                        "capturing it",
+                        bytes);
+        // queued work on s, or a captured graph, may still read the old
+        // buffer: it is retired, not freed
+        retire(ctx, {w->slot[slot].release()}, s);
         QOB_TRY(w->slot[slot].alloc(bytes));
     }
     p = w->slot[slot].p;
     return QOB_OK;
 }
 
-static hipStream_t strm(qob_ctx ctx, qob_stream s) { return s ? (hipStream_t)s : ctx->stream; }
+static hipStream_t strm(qob_ctx ctx, qob_stream s) {
+    if (!s) return ctx->stream;
+    const hipStream_t h = (hipStream_t)s;
+    if (h != ctx->stream && !ctx->foreign_streams) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (!ctx->lib_streams.count(h)) ctx->foreign_streams = true;
+    }
+    return h;
+}
 hipStream_t ctx_stream(qob_ctx ctx, qob_stream s) { return strm(ctx, s); }
 int ctx_device(qob_ctx ctx) { return ctx->device; }
 
@@ -1594,6 +1703,13 @@ static qob_status apply_lt_chain(const Apply &ap, OpNode &op, const void *in, vo
             QOB_TRY(dense_dev(*c.f->de, c.f->var ^ (ap.trans ? V_T : V_N), ap.dt, FM.A[i]));
         }
         FM.nmodes = (int)ops.size();
+        if (ok) {
+            // factors on sites 1, 2 (and 3): modes 1 and 2 chained in registers
+            hipError_t e = launch_lt_reg(ap.dt, FM, in, out, ap.trail, d2(a), d2(beta), ap.s);
+            if (e == hipSuccess) return QOB_OK;
+            if (e != hipErrorNotSupported) return fail(QOB_EHIP, "LazyTensor register-chain launch: %s", hipGetErrorString(e));
+            (void)hipGetLastError();
+        }
         if (ok && fused_supported(ap.dt, FM)) {
             hipError_t e = launch_lt_fused(ap.dt, FM, in, out, ap.trail, d2(a), d2(beta), ap.s);
             if (e != hipSuccess) return fail(QOB_EHIP, "fused LazyTensor launch: %s", hipGetErrorString(e));
@@ -2037,7 +2153,13 @@ qob_status qob_ctx_create(int device, qob_ctx *out) {
 
 qob_status qob_ctx_destroy(qob_ctx ctx) {
     if (!ctx) return QOB_OK;
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipDeviceSynchronize();
+    {
+        std::lock_guard<std::mutex> g(ctx->rmu);
+        for (Retired &R : ctx->retired)
+            for (hipEvent_t e : R.evs) (void)hipEventDestroy(e);
+        ctx->retired.clear();
+    }
     ctx->ws.clear();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2099,6 +2221,10 @@ qob_status qob_stream_create(qob_ctx ctx, qob_stream *out) {
     if (!ctx || !out) return fail(QOB_EARG, "null argument");
     hipStream_t s;
     QOB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->lib_streams.insert(s);
+    }
     *out = (qob_stream)s;
     return QOB_OK;
 }
@@ -2106,9 +2232,25 @@ qob_status qob_stream_create(qob_ctx ctx, qob_stream *out) {
 qob_status qob_stream_destroy(qob_ctx ctx, qob_stream s) {
     if (!ctx || !s) return fail(QOB_EARG, "null argument");
     QOB_HIP(hipStreamSynchronize((hipStream_t)s));
+    std::unique_ptr<Workspace> w;
     {
         std::lock_guard<std::mutex> g(ctx->mu);
-        ctx->ws.erase((hipStream_t)s);
+        auto it = ctx->ws.find((hipStream_t)s);
+        if (it != ctx->ws.end()) {
+            w = std::move(it->second);
+            ctx->ws.erase(it);
+        }
+        ctx->lib_streams.erase((hipStream_t)s);
+    }
+    if (w) {  // a live captured graph may still use this stream's workspace
+        std::vector<std::shared_ptr<void>> keep;
+        for (auto &b : w->slot) keep.push_back(b.release());
+        std::lock_guard<std::mutex> g(ctx->rmu);
+        if (ctx->live_graphs > 0) {
+            Retired R;
+            R.keep = std::move(keep);
+            ctx->retired.push_back(std::move(R));
+        }
     }
     QOB_HIP(hipStreamDestroy((hipStream_t)s));
     return QOB_OK;
@@ -2168,9 +2310,10 @@ qob_status qob_event_destroy(qob_event ev) {
 }
 
 // ------------------------------------------------------------- operators
-static qob_op wrap(std::shared_ptr<OpNode> n) {
+static qob_op wrap(qob_ctx ctx, std::shared_ptr<OpNode> n) {
     auto *h = new qob_op_s();
     h->node = std::move(n);
+    h->ctx = ctx;
     return h;
 }
 
@@ -2205,7 +2348,7 @@ qob_status qob_sparse_create(qob_ctx ctx, int64_t m, int64_t n, const int64_t *c
     node->m = m;
     node->n = n;
     node->sp = S;
-    *out = wrap(node);
+    *out = wrap(ctx, node);
     return QOB_OK;
 }
 
@@ -2223,7 +2366,7 @@ qob_status qob_dense_create(qob_ctx ctx, int64_t m, int64_t n, const qob_c64 *da
     node->m = m;
     node->n = n;
     node->de = D;
-    *out = wrap(node);
+    *out = wrap(ctx, node);
     return QOB_OK;
 }
 
@@ -2234,7 +2377,7 @@ qob_status qob_eye_create(qob_ctx ctx, int64_t m, int64_t n, qob_op *out) {
     node->kind = K_EYE;
     node->m = m;
     node->n = n;
-    *out = wrap(node);
+    *out = wrap(ctx, node);
     return QOB_OK;
 }
 
@@ -2286,13 +2429,13 @@ qob_status qob_lazyproduct_create(qob_ctx ctx, int nops, const qob_op *operators
     }
     node->m = node->terms.front()->m;
     node->n = node->terms.back()->n;
-    *out = wrap(node);
+    *out = wrap(ctx, node);
     return QOB_OK;
 }
 
 qob_status qob_op_adjoint(qob_ctx ctx, qob_op a, qob_op *out) {
     if (!ctx || !a || !out) return fail(QOB_EARG, "null argument");
-    *out = wrap(adjoint_node(a->node));
+    *out = wrap(ctx, adjoint_node(a->node));
     return QOB_OK;
 }
 
@@ -2325,7 +2468,7 @@ qob_status qob_lazytensor_create(qob_ctx ctx, int nsites, const int64_t *shape_l
         node->fsite.push_back((int)s);
         node->fop.push_back(factors[i]->node);
     }
-    *out = wrap(node);
+    *out = wrap(ctx, node);
     return QOB_OK;
 }
 
@@ -2357,7 +2500,7 @@ qob_status qob_lazysum_create(qob_ctx ctx, int64_t m, int64_t n, int nterms, con
         if (dc.empty()) dc.push_back(make_double2(0, 0));
         QOB_TRY(node->dcoef.upload(dc.data(), dc.size() * sizeof(double2)));
     }
-    *out = wrap(node);
+    *out = wrap(ctx, node);
     return QOB_OK;
 }
 
@@ -2365,18 +2508,25 @@ qob_status qob_lazysum_set_factors(qob_ctx ctx, qob_op sum, const qob_c64 *facto
     if (!ctx || !sum || sum->node->kind != K_LS) return fail(QOB_EARG, "not a LazySum");
     OpNode &n = *sum->node;
     if (!n.terms.empty() && !factors) return fail(QOB_EARG, "null factors");
-    // plans of the old coefficients may still be read by queued work
-    QOB_HIP(hipDeviceSynchronize());
-    std::lock_guard<std::mutex> g(n.mu);
-    for (size_t t = 0; t < n.terms.size(); ++t) n.coef[t] = cdv(factors[t]);
-    n.version++;
-    n.gplans.clear();
-    n.sums.clear();
-    n.qplans.clear();
-    n.dcoef_on = false;
-    std::vector<double2> dc(n.coef.size());
-    for (size_t t = 0; t < dc.size(); ++t) dc[t] = d2(n.coef[t]);
-    if (!dc.empty()) QOB_TRY(n.dcoef.upload(dc.data(), dc.size() * sizeof(double2)));
+    // plans of the old coefficients may still be read by queued work or by a
+    // captured graph: they are retired (freed once that work is done and no
+    // graph is alive); the next application builds plans of the new values
+    std::vector<std::shared_ptr<void>> old;
+    {
+        std::lock_guard<std::mutex> g(n.mu);
+        for (size_t t = 0; t < n.terms.size(); ++t) n.coef[t] = cdv(factors[t]);
+        n.version++;
+        for (auto &kv : n.gplans) old.push_back(kv.second);
+        for (auto &kv : n.sums) old.push_back(kv.second);
+        for (auto &kv : n.qplans) old.push_back(kv.second);
+        n.gplans.clear();
+        n.sums.clear();
+        n.qplans.clear();
+        // the device copy of the coefficients is read only while dcoef_on
+        // (refresh kernels); qob_lazysum_set_factors_d rewrites it
+        n.dcoef_on = false;
+    }
+    retire(ctx, std::move(old));
     return QOB_OK;
 }
 
@@ -2416,6 +2566,17 @@ qob_status qob_op_shape(qob_op a, int64_t *m, int64_t *n) {
 }
 
 qob_status qob_op_release(qob_op a) {
+    if (!a) return QOB_OK;
+    // a captured graph may still use the operator's plans and tables: while
+    // any graph of its context is alive the node is retired, not destroyed
+    if (a->ctx) {
+        bool graphs;
+        {
+            std::lock_guard<std::mutex> g(a->ctx->rmu);
+            graphs = a->ctx->live_graphs > 0;
+        }
+        if (graphs) retire(a->ctx, {std::shared_ptr<void>(a->node)});
+    }
     delete a;
     return QOB_OK;
 }
@@ -2598,7 +2759,7 @@ static int *tune_slot(const char *name) {
                {"fused_lt", &g_fused_lt},         {"qubit_enable", &g_qubit_enable},
                {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
-               {"qubit_split", &g_qubit_split},   {"mode_kernel", &g_mode_kernel},
+               {"qubit_split", &g_qubit_split},   {"mode_kernel", &g_mode_kernel}, {"fused_reg", &g_fused_reg},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
@@ -2636,7 +2797,7 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
          !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles") ||
-         !strcmp(name, "fused_resident")) &&
+         !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
@@ -2958,6 +3119,28 @@ qob_status qob_variance_ket(qob_ctx ctx, qob_dtype dt, qob_op a, const void *psi
     return QOB_OK;
 }
 
+qob_status qob_variance_op(qob_ctx ctx, qob_dtype dt, qob_op a, const void *rho, int64_t n, qob_c64 *out,
+                           qob_stream s) {
+    if (!ctx || !a || !out) return fail(QOB_EARG, "null argument");
+    OpNode &op = *a->node;
+    if (op.m != n || op.n != n)
+        return fail(QOB_EDIM, "variance: operator %lldx%lld on state %lldx%lld", (long long)op.m, (long long)op.n,
+                    (long long)n, (long long)n);
+    hipStream_t st = strm(ctx, s);
+    // x = op rho once; tr(op x) - tr(op rho)^2, both traces through expect
+    void *x;
+    QOB_TRY(ws_get(ctx, st, WS_VAR, (size_t)n * n * esize(dt), x));
+    Apply ap{ctx, st, dt, 1, n, false};
+    QOB_TRY(apply_op(ap, op, rho, x, cd(1, 0), cd(0, 0)));
+    qob_c64 e1, e2;
+    QOB_TRY(qob_expect_op(ctx, dt, a, rho, n, &e1, s));
+    QOB_TRY(qob_expect_op(ctx, dt, a, x, n, &e2, s));
+    const cd v = cdv(e2) - cdv(e1) * cdv(e1);
+    out->re = v.real();
+    out->im = v.imag();
+    return QOB_OK;
+}
+
 }  // extern "C"
 
 // ============================================================ batches
@@ -2978,6 +3161,7 @@ struct qob_batch_s {
 struct qob_graph_s {
     hipGraph_t g = nullptr;
     hipGraphExec_t ex = nullptr;
+    qob_ctx ctx = nullptr;
 };
 
 extern "C" {
@@ -3095,8 +3279,14 @@ qob_status qob_batch_info(qob_batch b, int64_t *total_elements, int *nstates) {
 }
 
 qob_status qob_batch_destroy(qob_batch b) {
-    if (b && b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
-    delete b;
+    if (!b) return QOB_OK;
+    if (!b->ctx) {
+        delete b;
+        return QOB_OK;
+    }
+    // applies may be queued on any stream, or captured in a live graph: the
+    // batch's tables are retired until that work is done and no graph is alive
+    retire(b->ctx, {std::shared_ptr<void>(b, [](void *p) { delete (qob_batch_s *)p; })});
     return QOB_OK;
 }
 
@@ -3114,6 +3304,12 @@ qob_status qob_graph_end(qob_ctx ctx, qob_stream s, qob_graph *out) {
         (void)hipGraphDestroy(G->g);
         return fail(QOB_EHIP, "hipGraphInstantiate failed");
     }
+    G->ctx = ctx;
+    {
+        // from now on nothing the graph may use is freed (see retire)
+        std::lock_guard<std::mutex> g(ctx->rmu);
+        ctx->live_graphs++;
+    }
     *out = G.release();
     return QOB_OK;
 }
@@ -3128,6 +3324,16 @@ qob_status qob_graph_destroy(qob_graph g) {
     if (!g) return QOB_OK;
     if (g->ex) (void)hipGraphExecDestroy(g->ex);
     if (g->g) (void)hipGraphDestroy(g->g);
+    if (g->ctx) {
+        qob_ctx ctx = g->ctx;
+        std::lock_guard<std::mutex> lk(ctx->rmu);
+        if (ctx->live_graphs > 0 && --ctx->live_graphs == 0 && !ctx->retired.empty()) {
+            // replays may still be in flight on any stream: wait for them, then
+            // free what was kept for the graphs
+            const bool synced = hipDeviceSynchronize() == hipSuccess;
+            reap_locked(ctx, synced);
+        }
+    }
     delete g;
     return QOB_OK;
 }

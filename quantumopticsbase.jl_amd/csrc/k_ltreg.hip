@@ -1,0 +1,258 @@
+// Dense LazyTensor with factors on sites 1, 2 (and 3) of a column: modes 1
+// and 2 chained in registers, one LDS exchange for mode 3.
+//
+// Reference: _tp_sum_matmul! (src/operators_lazytensor.jl:443-488) applies the
+// factors one mode at a time, each a full pass over the state.  For a column
+// x viewed as [i1 (16), i2 (d2), i3 (R)] (site 1 fastest) with factors A1
+// (16 x 16) on site 1, A2 (d2 x d2) on site 2 and optionally A3 (R x R) on
+// site 3, this kernel forms y = (A3 (x) A2 (x) A1) x as follows.  A block is
+// the 16 x d2 slice M = x[:, :, i3] (contiguous).
+//   mode 1, transposed: D1 = M^T A1^T (rows i2, columns i1'), MFMA A-operand
+//     the block read straight from HBM, B-operand A1^T from registers;
+//   mode 2: D2 = A2 D1, whose B-operand for k-step k is exactly register k % 4
+//     of D1 tile k / 4 (the f64 D tile row (lane/16) + 4 r, and the f32 row
+//     4 (lane/16) + r, are matched by the A2 fragments' k index), so the
+//     block never leaves registers between the two modes;
+//   mode 3 mixes the blocks: every wave writes its blocks' results into LDS
+//     planes once, and after a barrier the mode-3 products read them back and
+//     store the column to HBM.  Without a site-3 factor the blocks go from
+//     registers straight to HBM.
+// Per column that is one HBM read, one HBM write, one LDS write and one LDS
+// read of the column, and two barriers (the fused column kernel of
+// k_fused.hip writes the column back to LDS after every mode, with four
+// barriers).  Every complex product is four real MFMAs (qob_mfma.h).
+#include "qob_mfma.h"
+
+#include <atomic>
+
+namespace qob {
+
+namespace {
+
+__device__ __forceinline__ int pad16r(int i) { return i + (i >> 4); }
+
+constexpr int REG_NT = 512;  // 8 waves
+
+// k-index of mode-2 k-step ks, for the lane's k-row lk: the row of D1 tile
+// ks / 4 that register ks % 4 holds
+template <typename R> __device__ __forceinline__ int d1_row(int ks, int lk) {
+    const int t = ks >> 2, r = ks & 3;
+    return 16 * t + mf<R>::row(lk * 16, r);
+}
+
+// Modes 1 and 2 of one block (16 x D2 elements at xb): on return d2r/d2i
+// hold tile u of D2 = A2 (M^T A1^T), element (i1' = lane % 16,
+// i2' = 16 u + mf<R>::row(lane, r)) in register r.
+template <typename R, int D2, int LDA2>
+__device__ __forceinline__ void block_modes12(const cplx<R> *__restrict__ xb, const cplx<R> (&a1f)[4],
+                                              const cplx<R> *A2s, int lane, typename mf<R>::v4 (&d2r)[D2 / 16],
+                                              typename mf<R>::v4 (&d2i)[D2 / 16]) {
+    using T = cplx<R>;
+    using V = typename mf<R>::v4;
+    constexpr int NT2 = D2 / 16, KS2 = D2 / 4;
+    const int lr = lane & 15, lk = lane >> 4;
+    // mode 1: D1[t] = M^T A1^T over rows i2 = 16 t + ...
+    V d1r[NT2], d1i[NT2];
+    T x[NT2][4];
+#pragma unroll
+    for (int t = 0; t < NT2; ++t)
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) x[t][ks] = xb[(4 * ks + lk) + 16 * (16 * t + lr)];
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) {
+        d1r[t] = V{0, 0, 0, 0};
+        d1i[t] = V{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) cmma4<R>(x[t][ks], a1f[ks], d1r[t], d1i[t]);
+    }
+    // mode 2: D2[u] = A2 D1 (rows i2' = 16 u + ..., columns i1')
+#pragma unroll
+    for (int u = 0; u < NT2; ++u) {
+        d2r[u] = V{0, 0, 0, 0};
+        d2i[u] = V{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS2; ++ks) {
+        const int t = ks >> 2, r = ks & 3;
+        const T bv = mk((R)d1r[t][r], (R)d1i[t][r]);
+        const int i2 = d1_row<R>(ks, lk);
+#pragma unroll
+        for (int u = 0; u < NT2; ++u) cmma4<R>(A2s[(16 * u + lr) + i2 * LDA2], bv, d2r[u], d2i[u]);
+    }
+}
+
+// D2 = 16 x d2 blocks per column, nblk blocks per column; D3 > 0: the site-3
+// factor is nblk x nblk (nblk == D3), 0: no site-3 factor.
+template <typename R, int D2, int D3>
+__global__ void __launch_bounds__(REG_NT) k_lt_reg(const cplx<R> *__restrict__ A1, const cplx<R> *__restrict__ A2,
+                                                    const cplx<R> *__restrict__ A3, const cplx<R> *__restrict__ in,
+                                                    cplx<R> *__restrict__ out, int64_t ncols, int nblk,
+                                                    double2 alpha_, double2 beta_, int beta_zero) {
+    using T = cplx<R>;
+    using V = typename mf<R>::v4;
+    constexpr int BLK = 16 * D2;          // elements per block
+    constexpr int NT2 = D2 / 16;          // D2 tiles per block
+    constexpr int WPB = REG_NT / 64;
+    constexpr int LDA2 = D2 + 1;
+    extern __shared__ unsigned char lds_raw[];
+    T *A2s = (T *)lds_raw;                                   // A2, ld = d2 + 1
+    const int D = BLK * nblk;
+    const int Dp = pad16r(D) + 1;
+    R *yre = (R *)(A2s + D2 * LDA2);
+    R *yim = yre + Dp;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
+
+    for (int e = tid; e < D2 * D2; e += REG_NT) A2s[(e % D2) + (e / D2) * LDA2] = A2[e];
+    // A1^T B-fragments (k = i1, j = i1'): A1[i1' + 16 i1]
+    T a1f[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) a1f[ks] = A1[lr + 16 * (4 * ks + lk)];
+    // A3 A-fragments (i = k3', k = k3)
+    constexpr int NT3 = D3 > 0 ? D3 / 16 : 1, KS3 = D3 > 0 ? D3 / 4 : 1;
+    T a3f[NT3][KS3];
+    if constexpr (D3 > 0) {
+#pragma unroll
+        for (int v = 0; v < NT3; ++v)
+#pragma unroll
+            for (int ks = 0; ks < KS3; ++ks) a3f[v][ks] = A3[(16 * v + lr) + (int64_t)D3 * (4 * ks + lk)];
+    }
+    __syncthreads();  // A2 staged
+
+    for (int64_t col = blockIdx.x; col < ncols; col += gridDim.x) {
+        const T *xc = in + col * (int64_t)D;
+        T *oc = out + col * (int64_t)D;
+        V d2r[NT2], d2i[NT2];
+        if constexpr (D3 == 0) {
+            // no site-3 factor: every block goes from registers to HBM
+            for (int blk = wave; blk < nblk; blk += WPB) {
+                block_modes12<R, D2, LDA2>(xc + (int64_t)blk * BLK, a1f, A2s, lane, d2r, d2i);
+#pragma unroll
+                for (int u = 0; u < NT2; ++u)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        T *o = oc + (int64_t)blk * BLK + lr + 16 * (16 * u + mf<R>::row(lane, r));
+                        T v = cmul(alpha, mk((R)d2r[u][r], (R)d2i[u][r]));
+                        if (!beta_zero) v = cfma(beta, *o, v);
+                        st_stream(o, v);
+                    }
+            }
+        } else {
+            // the wave's first block is formed while other waves may still be
+            // reading the previous column's planes; it is written after the
+            // barrier, the wave's other blocks straight after they are formed
+            const bool first = wave < nblk;
+            if (first) block_modes12<R, D2, LDA2>(xc + (int64_t)wave * BLK, a1f, A2s, lane, d2r, d2i);
+            __syncthreads();  // every wave is done reading the previous column's planes
+            for (int blk = wave; blk < nblk; blk += WPB) {
+                if (blk != wave) block_modes12<R, D2, LDA2>(xc + (int64_t)blk * BLK, a1f, A2s, lane, d2r, d2i);
+#pragma unroll
+                for (int u = 0; u < NT2; ++u)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int e = blk * BLK + lr + 16 * (16 * u + mf<R>::row(lane, r));
+                        yre[pad16r(e)] = d2r[u][r];
+                        yim[pad16r(e)] = d2i[u][r];
+                    }
+            }
+            __syncthreads();  // the column's planes are complete
+            // mode 3: Z^T (k3' x e) = A3 Y^T over groups of 16 elements e
+            for (int g = wave; g < BLK / 16; g += WPB) {
+                const int e0 = 16 * g;
+                V zr[NT3], zi[NT3];
+#pragma unroll
+                for (int v = 0; v < NT3; ++v) {
+                    zr[v] = V{0, 0, 0, 0};
+                    zi[v] = V{0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int ks = 0; ks < KS3; ++ks) {
+                    const int idx = pad16r(e0 + lr + BLK * (4 * ks + lk));
+                    const T yv = mk(yre[idx], yim[idx]);
+#pragma unroll
+                    for (int v = 0; v < NT3; ++v) cmma4<R>(a3f[v][ks], yv, zr[v], zi[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < NT3; ++v)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        T *o = oc + e0 + lr + (int64_t)BLK * (16 * v + mf<R>::row(lane, r));
+                        T val = cmul(alpha, mk((R)zr[v][r], (R)zi[v][r]));
+                        if (!beta_zero) val = cfma(beta, *o, val);
+                        st_stream(o, val);
+                    }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int g_fused_reg = 1;  // 1: this kernel where the factors' sites and sizes fit it
+
+size_t lt_reg_lds(qob_dtype dt, int d2, int D) {
+    const size_t r = dt == QOB_C64 ? 8 : 4;
+    const size_t dp = (size_t)(D + (D >> 4) + 1);
+    return (size_t)d2 * (d2 + 1) * 2 * r + 2 * dp * r;
+}
+
+// Launch when the factors sit on sites 1 and 2 (and 3, the last site) with
+// d1 = 16, d2 in {16, 32} and d3 in {16, 32} (not both 32); returns
+// hipErrorNotSupported otherwise (the caller uses the fused column kernel).
+hipError_t launch_lt_reg(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols, double2 alpha,
+                         double2 beta, hipStream_t s) {
+    if (!g_fused_reg || M.nmodes < 2 || M.nmodes > 3) return hipErrorNotSupported;
+    if (M.n1[0] != 1 || M.d[0] != 16 || M.n1[1] != 16 || (M.d[1] != 16 && M.d[1] != 32)) return hipErrorNotSupported;
+    const int d2 = M.d[1], blk = 16 * d2;
+    if (M.D % blk) return hipErrorNotSupported;
+    const int nblk = M.D / blk;
+    int d3 = 0;
+    if (M.nmodes == 3) {
+        d3 = M.d[2];
+        if (M.n1[2] != blk || (d3 != 16 && d3 != 32) || d3 != nblk) return hipErrorNotSupported;
+    }
+    const size_t lds = d3 ? lt_reg_lds(dt, d2, M.D) : (size_t)d2 * (d2 + 1) * (dt == QOB_C64 ? 16 : 8);
+    if (lds > 160 * 1024) return hipErrorNotSupported;
+    if (ncols == 0) return hipSuccess;
+    const int bz = (beta.x == 0.0 && beta.y == 0.0);
+    const int ncu = device_cu_count();
+#define QOB_LR(RR, DD2, DD3)                                                                                    \
+    do {                                                                                                        \
+        auto k = k_lt_reg<RR, DD2, DD3>;                                                                        \
+        static std::atomic<int> nres{0};                                                                        \
+        if (!nres.load()) {                                                                                     \
+            hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                               160 * 1024);                                                     \
+            if (e != hipSuccess) return e;                                                                      \
+            int nb = 0;                                                                                         \
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, REG_NT, lds);                              \
+            if (e != hipSuccess) return e;                                                                      \
+            nres.store(nb > 0 ? nb : 1);                                                                        \
+        }                                                                                                       \
+        const unsigned g = (unsigned)std::min<int64_t>(ncols, (int64_t)ncu * nres.load());                      \
+        hipLaunchKernelGGL(k, dim3(g), dim3(REG_NT), lds, s, (const cplx<RR> *)M.A[0], (const cplx<RR> *)M.A[1], \
+                           (const cplx<RR> *)(d3 ? M.A[2] : nullptr), (const cplx<RR> *)in, (cplx<RR> *)out,    \
+                           ncols, nblk, alpha, beta, bz);                                                       \
+        return hipGetLastError();                                                                               \
+    } while (0)
+    // (d2, d3) = (32, 32) is left to the fused column kernel: its eight
+    // mode-3 fragment pairs per lane do not fit next to the rest
+    if (dt == QOB_C64) {
+        if (d2 == 16 && d3 == 0) QOB_LR(double, 16, 0);
+        if (d2 == 16 && d3 == 16) QOB_LR(double, 16, 16);
+        if (d2 == 16 && d3 == 32) QOB_LR(double, 16, 32);
+        if (d2 == 32 && d3 == 0) QOB_LR(double, 32, 0);
+        if (d2 == 32 && d3 == 16) QOB_LR(double, 32, 16);
+    } else {
+        if (d2 == 16 && d3 == 0) QOB_LR(float, 16, 0);
+        if (d2 == 16 && d3 == 16) QOB_LR(float, 16, 16);
+        if (d2 == 16 && d3 == 32) QOB_LR(float, 16, 32);
+        if (d2 == 32 && d3 == 0) QOB_LR(float, 32, 0);
+        if (d2 == 32 && d3 == 16) QOB_LR(float, 32, 16);
+    }
+#undef QOB_LR
+    return hipErrorNotSupported;
+}
+
+}  // namespace qob

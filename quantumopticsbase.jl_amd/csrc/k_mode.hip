@@ -32,6 +32,7 @@
 //                 of F split over workgroups, deterministic two-stage sum.
 #include "qob_mfma.h"
 
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <unordered_set>
@@ -132,8 +133,9 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_fib(const cplx<R> *__restrict_
 
 // ---------------------------------------------------------- k_mode_chunk
 // Chunks of cpc consecutive i3 blocks (cpc * n1 * d elements, contiguous).
+constexpr int CHUNK_NT = 256;  // threads per workgroup: small chunks, several workgroups per CU
 template <typename R, int NT16>
-__global__ void __launch_bounds__(MODE_NT) k_mode_chunk(const cplx<R> *__restrict__ F, int d_out, int d_in,
+__global__ void __launch_bounds__(CHUNK_NT) k_mode_chunk(const cplx<R> *__restrict__ F, int d_out, int d_in,
                                                          const cplx<R> *__restrict__ in, cplx<R> *__restrict__ out,
                                                          int n1, int64_t n3, int cpc, int64_t nchunks,
                                                          double2 alpha_, double2 beta_, int beta_zero) {
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_chunk(const cplx<R> *__restric
         const int nb = (int)(n3 - i30 < cpc ? n3 - i30 : cpc);
         const int ne_in = nb * n1 * d_in, ne_out = nb * n1 * d_out;
         const T *src = in + i30 * n1 * d_in;
-        for (int e = tid; e < ne_in; e += MODE_NT) {
+        for (int e = tid; e < ne_in; e += CHUNK_NT) {
             const T v = src[e];
             ire[pad16(e)] = v.x;
             iim[pad16(e)] = v.y;
@@ -164,7 +166,7 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_chunk(const cplx<R> *__restric
         // input planes loaded; the previous chunk's output planes are copied out
         __syncthreads();
         const int nf = nb * n1, ng = (nf + 15) >> 4;
-        for (int g = wave; g < ng; g += MODE_NT / 64) {
+        for (int g = wave; g < ng; g += CHUNK_NT / 64) {
             const int f = g * 16 + lr;
             const bool ok = f < nf;
             const int i3l = ok ? f / n1 : 0, i1 = ok ? f - i3l * n1 : 0;
@@ -214,7 +216,7 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_chunk(const cplx<R> *__restric
         // output planes complete; every wave is done reading the input planes
         __syncthreads();
         T *dst = out + i30 * n1 * d_out;
-        for (int e = tid; e < ne_out; e += MODE_NT) {
+        for (int e = tid; e < ne_out; e += CHUNK_NT) {
             T v = cmul(alpha, mk(ore[pad16(e)], oim[pad16(e)]));
             if (!beta_zero) v = cfma(beta, dst[e], v);
             st_stream(dst + e, v);
@@ -433,6 +435,21 @@ __global__ void __launch_bounds__(256) k_zgemv_t(const cplx<R> *__restrict__ A, 
     }
 }
 
+// Workgroups of a kernel resident per CU at the given block size and LDS
+// (the runtime's occupancy query, cached per kernel and LDS size).
+template <typename K> int resident_per_cu(K kfn, int nt, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, size_t>, int> cache;
+    std::lock_guard<std::mutex> g(mu);
+    auto key = std::make_pair((const void *)kfn, lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kfn, nt, lds) != hipSuccess || nb < 1) nb = 1;
+    cache.emplace(key, nb);
+    return nb;
+}
+
 // Dynamic LDS above 64 KiB needs the kernel's limit raised once (to the full
 // 160 KiB, so later launches with any size are covered).
 template <typename K> hipError_t raise_lds(K kfn, size_t bytes) {
@@ -525,14 +542,14 @@ hipError_t launch_zgemv_t(qob_dtype dt, const void *A, int64_t k, int64_t m, con
     return hipGetLastError();
 }
 
-// chunk of i3 blocks for k_mode_chunk: up to 64 KiB of planes (in + out),
-// at least one block, and where that fits a whole number of 8 fibre groups
-// (one per wave), else of single fibre groups
+// chunk of i3 blocks for k_mode_chunk: up to 32 KiB of planes (in + out),
+// at least one block, and where that fits a whole number of fibre groups
+// per wave (4 waves), else of single fibre groups
 static int mode_chunk_blocks(int64_t n1, int64_t d_in, int64_t d_out, size_t esz) {
     const int64_t per = n1 * (d_in + d_out) * 2 * (int64_t)esz;  // plane bytes per i3 block
-    int64_t c = std::max<int64_t>(1, (64 * 1024) / std::max<int64_t>(per, 1));
-    const int64_t u8 = 128 / std::gcd<int64_t>(n1, 128), u1 = 16 / std::gcd<int64_t>(n1, 16);
-    if (c >= u8) c -= c % u8;
+    int64_t c = std::max<int64_t>(1, (32 * 1024) / std::max<int64_t>(per, 1));
+    const int64_t uw = 64 / std::gcd<int64_t>(n1, 64), u1 = 16 / std::gcd<int64_t>(n1, 16);
+    if (c >= uw) c -= c % uw;
     else if (c >= u1) c -= c % u1;
     return (int)c;
 }
@@ -569,13 +586,13 @@ hipError_t launch_mode(qob_dtype dt, const void *F, int64_t d_out, int64_t d_in,
     const int ncu = device_cu_count();
     if (n1 >= 16) {
         const int64_t gpi = (n1 + 15) / 16, ng = gpi * n3;
-        const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, (160 * 1024) / (int64_t)fbytes));
-        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((ng + 7) / 8, (int64_t)ncu * per_cu));
 #define QOB_MF(RR, NT)                                                                                          \
     do {                                                                                                        \
         auto k = k_mode_fib<RR, NT>;                                                                            \
         hipError_t e = raise_lds(k, fbytes);                                                                    \
         if (e != hipSuccess) return e;                                                                          \
+        const int64_t grid = std::max<int64_t>(                                                                 \
+            1, std::min<int64_t>((ng + 7) / 8, (int64_t)ncu * resident_per_cu(k, MODE_NT, fbytes)));           \
         hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(MODE_NT), fbytes, s, (const cplx<RR> *)F, (int)d_out, \
                            (int)d_in, (const cplx<RR> *)in, (cplx<RR> *)out, (int)n1, (int)gpi, ng, alpha, beta, bz); \
         return hipGetLastError();                                                                               \
@@ -600,14 +617,14 @@ hipError_t launch_mode(qob_dtype dt, const void *F, int64_t d_out, int64_t d_in,
     const size_t pout = (size_t)(cpc * n1 * d_out + ((cpc * n1 * d_out) >> 4) + 1);
     const size_t lds = fbytes + 2 * (pin + pout) * esz;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, (160 * 1024) / (int64_t)lds));
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(nch, (int64_t)ncu * per_cu));
 #define QOB_MC(RR, NT)                                                                                        \
     do {                                                                                                      \
         auto k = k_mode_chunk<RR, NT>;                                                                        \
         hipError_t e = raise_lds(k, lds);                                                                     \
         if (e != hipSuccess) return e;                                                                        \
-        hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(MODE_NT), lds, s, (const cplx<RR> *)F, (int)d_out,  \
+        const int64_t grid =                                                                                  \
+            std::max<int64_t>(1, std::min<int64_t>(nch, (int64_t)ncu * resident_per_cu(k, CHUNK_NT, lds)));   \
+        hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(CHUNK_NT), lds, s, (const cplx<RR> *)F, (int)d_out,  \
                            (int)d_in, (const cplx<RR> *)in, (cplx<RR> *)out, (int)n1, n3, cpc, nch, alpha, beta, bz); \
         return hipGetLastError();                                                                             \
     } while (0)

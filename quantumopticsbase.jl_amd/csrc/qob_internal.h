@@ -270,6 +270,12 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
                            double2 alpha, double2 beta, hipStream_t s, int64_t tile_ld = 0,
                            int64_t tile_nrb = 1);
 
+// Factors on sites 1 and 2 (d1 = 16, d2 in {16, 32}) and optionally on the
+// last site 3 (d3 in {16, 32}) of ncols columns: modes 1 and 2 chained in
+// registers (k_ltreg.hip).  hipErrorNotSupported when the shape does not fit.
+hipError_t launch_lt_reg(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols,
+                         double2 alpha, double2 beta, hipStream_t s);
+
 // Fused reductions over a square gather plan (FP64 partials, one per block):
 //   mode 0: sum_J conj(u[J]) (H v)[J]          (expect / bilinear on a Ket)
 //   mode 1: sum_J (H V[:, J])[J], V n x n       (tr(H V), expect on rho)

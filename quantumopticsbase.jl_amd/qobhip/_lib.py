@@ -123,6 +123,7 @@ _PROTOS = {
     "qob_expect_ket": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
     "qob_expect_op": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
     "qob_variance_ket": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
+    "qob_variance_op": (_i32, [_vp, _i32, _vp, _vp, _i64, _P(qob_c64), _vp]),
     "qob_batch_create": (_i32, [_vp, _i32, _P(_vp), _i32, _P(_vp)]),
     "qob_batch_bind": (_i32, [_vp, _vp, _i32, _P(_vp), _P(_vp)]),
     "qob_batch_apply": (_i32, [_vp, _vp, _i32, qob_c64, qob_c64, qob_c64, _vp]),

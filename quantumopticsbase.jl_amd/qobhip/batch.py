@@ -12,6 +12,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import c64, call
+from . import core as _core
 from .core import DeviceArray, Ket, _dt, context
 
 
@@ -53,10 +54,12 @@ class Batch:
 
     def apply(self, slot: int, alpha=1.0, beta=0.0, gamma=0.0, stream=None) -> None:
         """out_j = alpha op_j in_j + gamma in_j + beta out_j for every j."""
+        _core._note_capture(self, self._bufs.get(slot))
         call("qob_batch_apply", context().h, self.h, slot, c64(alpha), c64(beta), c64(gamma), stream)
 
     def axpy(self, slot: int, a, stream=None) -> None:
         """out_j += a in_j for every j."""
+        _core._note_capture(self, self._bufs.get(slot))
         call("qob_batch_axpy", context().h, self.h, slot, c64(a), stream)
 
     def __del__(self):
@@ -70,17 +73,28 @@ class Batch:
 class Graph:
     """Capture library calls issued inside ``with Graph() as g:`` into a HIP
     graph; ``g.launch()`` replays them.  Every captured call must have run
-    once before capture."""
+    once before capture.
 
-    def __init__(self, stream=None):
+    A replay reads the operators, batches and device arrays that the captured
+    calls used, at the addresses they had during capture.  The Graph keeps a
+    reference to every one of them that went through this package (mul_,
+    Batch.apply / axpy, set_factors_device) for as long as it lives; arrays
+    reached only through raw library calls can be added with ``keep``.  The
+    library itself keeps its own workspaces, plans, operator tables and
+    batch tables alive while any graph of the context exists."""
+
+    def __init__(self, stream=None, keep=()):
         self.stream = stream
         self.h = None
+        self.keep = list(keep)
 
     def __enter__(self):
         call("qob_graph_begin", context().h, self.stream)
+        _core._CAPTURE_KEEP.append(self.keep)
         return self
 
     def __exit__(self, exc_type, exc, tb):
+        _core._CAPTURE_KEEP.pop()
         h = C.c_void_p()
         call("qob_graph_end", context().h, self.stream, C.byref(h))
         self.h = h

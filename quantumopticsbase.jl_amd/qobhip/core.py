@@ -56,6 +56,18 @@ class Context:
         return s
 
 
+# Objects a HIP graph being captured uses: while a Graph (batch.py) captures,
+# the entry points of this mirror add the operators, batches and device
+# arrays of every call to the innermost open list; the Graph keeps them alive
+# for as long as it can be replayed.
+_CAPTURE_KEEP: list = []
+
+
+def _note_capture(*objs) -> None:
+    if _CAPTURE_KEEP:
+        _CAPTURE_KEEP[-1].extend(o for o in objs if o is not None)
+
+
 def context() -> Context:
     global _CTX
     if _CTX is None:
@@ -529,6 +541,7 @@ class LazySum(AbstractOperator):
         doubles at ``dfactors``, a DeviceArray or a raw pointer), stream-ordered
         and capturable; the host copy ``factors`` is not updated."""
         ptr = dfactors.ptr if isinstance(dfactors, DeviceArray) else dfactors
+        _note_capture(self, dfactors)
         call("qob_lazysum_set_factors_d", context().h, self._handle(), ptr, stream)
 
 
@@ -630,6 +643,7 @@ def mul_(result, a, b, alpha=1.0, beta=0.0):
         return result
     ctx = context()
     al, be = c64(alpha), c64(beta)
+    _note_capture(result, a, b)
     if isinstance(result, Ket):
         if not (isinstance(b, Ket) and isinstance(a, AbstractOperator)):
             raise MethodError(f"no method mul!(::Ket, ::{type(a).__name__}, ::{type(b).__name__})")
@@ -823,8 +837,15 @@ def variance(op: AbstractOperator, state) -> complex:
     variance(op, rho) = tr(op op rho) - tr(op rho)^2 with x = op rho formed
     once and both traces taken by expect (fused where the operator allows)."""
     if isinstance(state, Operator):
-        x = op * state
-        return expect(op, x) - expect(op, state) ** 2
+        if isinstance(op, Operator):
+            x = op * state
+            return expect(op, x) - expect(op, state) ** 2
+        if len(state.basis_l) != len(state.basis_r):
+            raise DimensionMismatch("variance on a non-square operator")
+        out = qob_c64()
+        call("qob_variance_op", context().h, _dt(state.dtype), op._handle(), state.data.ptr, len(state.basis_l),
+             C.byref(out), None)
+        return complex(out.re, out.im)
     if not isinstance(state, Ket):
         raise MethodError(f"variance not defined for {type(state).__name__}")
     if isinstance(op, Operator):

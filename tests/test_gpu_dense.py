@@ -270,3 +270,56 @@ def test_mixed_element_types_like_reference(kind):
     R = Q.Operator(gb(7), br, R0, np.complex64)
     Q.mul_(R, Q.Operator(gb(7), bl, B, np.complex128), qop, 1.0, -1.0)
     assert rel(R.to_numpy(), want) < 1e-6
+
+
+# factors on sites 1, 2 (and 3): the register-chained kernel (k_ltreg.hip)
+REG_CASES = [
+    ((16, 16), (1, 2)),           # no site-3 factor, d2 = 16
+    ((16, 32), (1, 2)),           # no site-3 factor, d2 = 32
+    ((16, 32, 4), (1, 2)),        # no site-3 factor, 4 blocks per column
+    ((16, 16, 16), (1, 2, 3)),
+    ((16, 32, 16), (1, 2, 3)),    # the 8192-element column of the dense benchmark
+    ((16, 16, 32), (1, 2, 3)),
+    ((16, 32, 32), (1, 2, 3)),    # d2 = d3 = 32: falls back to the fused column kernel
+]
+
+
+@pytest.mark.parametrize("case", range(len(REG_CASES)))
+@pytest.mark.parametrize("dtype", DTS)
+def test_register_chain_kernel(case, dtype):
+    """Modes 1 and 2 chained in registers, mode 3 through one LDS exchange:
+    against the oracle, and against the fused column kernel (fused_reg = 0),
+    with (alpha, beta) general and beta = 0 over NaN."""
+    dims, idx = REG_CASES[case]
+    g = np.random.default_rng(500 + case)
+    facs = [cg(g, d, d) for d in dims[:len(idx)]]
+    lt = Q.LazyTensor(comp(dims), comp(dims), idx, tuple(Q.DenseOperator(gb(d), gb(d), A)
+                                                         for d, A in zip(dims, facs)))
+    olt = O.OLazyTensor(dims, dims, idx, tuple(O.ODense(A) for A in facs), 1.0)
+    D = int(np.prod(dims))
+    ncol = 37
+    B = cg(g, D, ncol)
+    R0 = cg(g, D, ncol)
+    al, be = complex(0.4, -0.9), complex(0.25, 0.5)
+    want = R0.copy()
+    O.mul_op_left(want, olt, B, al, be)
+    outs = []
+    for reg in (1, 0):
+        old = _tune("fused_reg", reg)
+        try:
+            R = Q.Operator(lt.basis_l, gb(ncol), R0, dtype)
+            Q.mul_(R, lt, Q.Operator(lt.basis_r, gb(ncol), B, dtype), al, be)
+            outs.append(R.to_numpy())
+            assert rel(outs[-1], want) < TOLS[dtype], reg
+            Rn = Q.Operator(lt.basis_l, gb(ncol), np.full((D, ncol), np.nan + 0j), dtype)
+            Q.mul_(Rn, lt, Q.Operator(lt.basis_r, gb(ncol), B, dtype), 1.0, 0.0)
+            w0 = np.zeros((D, ncol), complex)
+            O.mul_op_left(w0, olt, B, 1.0, 0.0)
+            assert np.isfinite(Rn.to_numpy()).all() and rel(Rn.to_numpy(), w0) < TOLS[dtype]
+            # a Ket is a single column
+            k = Q.Ket(lt.basis_l, np.zeros(D, complex), dtype)
+            Q.mul_(k, lt, Q.Ket(lt.basis_r, B[:, 0], dtype))
+            assert rel(k.to_numpy(), w0[:, 0]) < TOLS[dtype]
+        finally:
+            _tune("fused_reg", old)
+    assert rel(outs[0], outs[1]) < TOLS[dtype]
