@@ -413,38 +413,60 @@ def run_ket10(args):
 
 
 # --------------------------------------------------------------- dense8192
+def _events(ctx, n):
+    from qobhip import _lib
+    evs = []
+    for _ in range(n):
+        e = C.c_void_p()
+        _lib.call("qob_event_create", ctx.h, C.byref(e))
+        evs.append(e)
+    return evs
+
+
+def _elapsed(a, b):
+    from qobhip import _lib
+    v = C.c_float()
+    _lib.call("qob_event_elapsed", a, b, C.byref(v))
+    return float(v.value)
+
+
 def timed_events(ctx, step, steps: int, warmup: int):
     """W untimed warmup steps, then exactly K steps between synchronisations.
-    An event on the library's stream before the first timed step and after
-    each one gives the kernel time over the timed region itself.  Returns
-    (host seconds, event ms over the K steps, list of K per-step event ms)."""
+    Events on the library's stream before the first and after the last timed
+    step give the kernel time over the timed region itself (nothing else is
+    enqueued between the steps).  Returns (host seconds, event ms over the K
+    steps)."""
     from qobhip import _lib
     for _ in range(warmup):
         step()
     ctx.synchronize()
-    evs = []
-    for _ in range(steps + 1):
-        e = C.c_void_p()
-        _lib.call("qob_event_create", ctx.h, C.byref(e))
-        evs.append(e)
-    _lib.call("qob_event_record", ctx.h, evs[0], ctx.stream)
+    e0, e1 = _events(ctx, 2)
+    _lib.call("qob_event_record", ctx.h, e0, ctx.stream)
     t0 = time.perf_counter()
-    for i in range(steps):
+    for _ in range(steps):
+        step()
+    _lib.call("qob_event_record", ctx.h, e1, ctx.stream)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    total = _elapsed(e0, e1)
+    for e in (e0, e1):
+        _lib.load().qob_event_destroy(e)
+    return el, total
+
+
+def latency_events(ctx, step, n: int):
+    """Per-apply device latency: an event after every one of n steps."""
+    from qobhip import _lib
+    evs = _events(ctx, n + 1)
+    _lib.call("qob_event_record", ctx.h, evs[0], ctx.stream)
+    for i in range(n):
         step()
         _lib.call("qob_event_record", ctx.h, evs[i + 1], ctx.stream)
     ctx.synchronize()
-    el = time.perf_counter() - t0
-
-    def ms(a, b):
-        v = C.c_float()
-        _lib.call("qob_event_elapsed", a, b, C.byref(v))
-        return float(v.value)
-
-    total = ms(evs[0], evs[-1])
-    per = [ms(evs[i], evs[i + 1]) for i in range(steps)]
+    per = [_elapsed(evs[i], evs[i + 1]) for i in range(n)]
     for e in evs:
         _lib.load().qob_event_destroy(e)
-    return el, total, per
+    return per
 
 
 def sparse6400_figures(steps: int = 20):
@@ -506,8 +528,9 @@ def run_dense8192(args):
     log(f"bench: dense8192 {args.dtype} warmup {args.warmup}, timing {args.steps} steps")
     # one step is one left apply, which is one launch of the fused kernel:
     # the events around the timed steps give that kernel's average duration
-    el, ev_ms, per = timed_events(ctx, left, args.steps, args.warmup)
+    el, ev_ms = timed_events(ctx, left, args.steps, args.warmup)
     ms_fused = ev_ms / args.steps
+    per = latency_events(ctx, left, max(20, min(200, args.steps)))
     ev = EventTimer(ctx.h, ctx.stream)
     right()
     ctx.synchronize()
@@ -516,7 +539,7 @@ def run_dense8192(args):
     # (_tp_sum_matmul!, one pass over the state per factor), same stream
     _lib.call("qob_tune_set", b"fused_lt", 0)
     plain_steps = max(3, min(10, args.steps))
-    el_plain, ev_plain, _ = timed_events(ctx, left, plain_steps, 1)
+    el_plain, ev_plain = timed_events(ctx, left, plain_steps, 2)
     ms_chain = ev_plain / plain_steps
     _lib.call("qob_tune_set", b"fused_lt", 1)
     plain_aps = plain_steps / el_plain
@@ -563,13 +586,15 @@ def run_dense8192(args):
                          f"src/operators_lazytensor.jl:281-488 (reshape/permute + BLAS GEMM per mode), ComplexF64, "
                          f"numpy BLAS on {nthr} threads, {t:.1f} s"}
         del rho_h, res
-    # with these factors (at most 32 wide) the kernel forms each complex
-    # product from three real MFMAs (Gauss's form) in ComplexF32, and in
-    # ComplexF64 when its factors are resident with one fibre group per wave;
-    # "achieved" counts the algorithmic 8 flops per complex multiply-add
-    gauss = dt == np.complex64 or (T.get("fused_resident") == 1 and T.get("fused_gpw") in (0, 1))
-    mfma_flops = flops * 3 / 4 if gauss else flops
-    roof = {"bound": "mfma", "kernel": "k_lt_fused", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
+    # factors on sites 1, 2, 3 of widths (16, 32, 16): the register-chained
+    # kernel k_lt_reg, four real MFMAs per complex product in both element
+    # types, so the MFMAs execute exactly the algorithmic 8 flops per complex
+    # multiply-add.  With fused_reg = 0 the fused column kernel runs; its
+    # ComplexF32 form uses three real products (Gauss's form).
+    reg = T.get("fused_reg") == 1
+    mfma_flops = flops if reg or dt == np.complex128 else flops * 3 / 4
+    roof = {"bound": "mfma", "kernel": "k_lt_reg" if reg else "k_lt_fused", "achieved": tflops, "peak": peak,
+            "unit": "TFLOP/s",
             "frac": tflops / peak, "traffic": traffic_of(f"lt_fused_{args.dtype}"),
             "algorithmic_flops_per_launch": flops, "mfma_flops_per_launch": mfma_flops,
             "executed_tflops": mfma_flops / (ms_fused * 1e-3) / 1e12,
@@ -581,7 +606,7 @@ def run_dense8192(args):
                           f"8192x8192 {args.dtype} Ginibre trace-1 density matrix; 1 step = 1 apply",
               "N": N, "dims": dims,
               "kernel_ms": {"fused_left": ms_fused, "chain_left": ms_chain, "right_tiles": ms_right},
-              "latency_ms_per_apply": {"p50": float(ps[0]), "p99": float(ps[1])},
+              "latency_ms_per_apply": {"p50": float(ps[0]), "p99": float(ps[1]), "applies": len(per)},
               "plain_gpu_path": {"applies_per_s": plain_aps,
                                  "what": "mode-by-mode chain on the GPU (_tp_sum_matmul!, one pass per factor)"},
               **extra}
@@ -740,7 +765,7 @@ def run_serve(args):
 
 WORKLOADS = {"sparse6400": run_sparse6400, "ket10": run_ket10, "dense8192": run_dense8192,
              "batch256": run_batch256, "serve": run_serve}
-DEFAULT_STEPS = {"sparse6400": (200, 10), "ket10": (2000, 100), "dense8192": (40, 5), "batch256": (200, 10),
+DEFAULT_STEPS = {"sparse6400": (200, 10), "ket10": (2000, 100), "dense8192": (100, 20), "batch256": (200, 10),
                  "serve": (0, 0)}
 
 

@@ -40,11 +40,22 @@ template <typename R> __device__ __forceinline__ int d1_row(int ks, int lk) {
     return 16 * t + mf<R>::row(lk * 16, r);
 }
 
-// Modes 1 and 2 of one block (16 x D2 elements at xb): on return d2r/d2i
+// The mode-1 A-fragments of one block (16 x D2 elements at xb): element
+// (i1 = 4 ks + lane / 16, i2 = 16 t + lane % 16), read straight from HBM.
+template <typename R, int D2>
+__device__ __forceinline__ void block_load(const cplx<R> *__restrict__ xb, int lane, cplx<R> (&x)[D2 / 16][4]) {
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < D2 / 16; ++t)
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) x[t][ks] = xb[(4 * ks + lk) + 16 * (16 * t + lr)];
+}
+
+// Modes 1 and 2 of one block from its loaded fragments: on return d2r/d2i
 // hold tile u of D2 = A2 (M^T A1^T), element (i1' = lane % 16,
 // i2' = 16 u + mf<R>::row(lane, r)) in register r.
 template <typename R, int D2, int LDA2>
-__device__ __forceinline__ void block_modes12(const cplx<R> *__restrict__ xb, const cplx<R> (&a1f)[4],
+__device__ __forceinline__ void block_modes12(const cplx<R> (&x)[D2 / 16][4], const cplx<R> (&a1f)[4],
                                               const cplx<R> *A2s, int lane, typename mf<R>::v4 (&d2r)[D2 / 16],
                                               typename mf<R>::v4 (&d2i)[D2 / 16]) {
     using T = cplx<R>;
@@ -53,11 +64,6 @@ __device__ __forceinline__ void block_modes12(const cplx<R> *__restrict__ xb, co
     const int lr = lane & 15, lk = lane >> 4;
     // mode 1: D1[t] = M^T A1^T over rows i2 = 16 t + ...
     V d1r[NT2], d1i[NT2];
-    T x[NT2][4];
-#pragma unroll
-    for (int t = 0; t < NT2; ++t)
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) x[t][ks] = xb[(4 * ks + lk) + 16 * (16 * t + lr)];
 #pragma unroll
     for (int t = 0; t < NT2; ++t) {
         d1r[t] = V{0, 0, 0, 0};
@@ -120,14 +126,29 @@ __global__ void __launch_bounds__(REG_NT) k_lt_reg(const cplx<R> *__restrict__ A
     }
     __syncthreads();  // A2 staged
 
-    for (int64_t col = blockIdx.x; col < ncols; col += gridDim.x) {
+    // ComplexF64: global loads run one block ahead, a block's fragments
+    // requested before the previous block's MFMAs (and the first block of the
+    // next column before this column's mode 3).  ComplexF32 loads each block
+    // just before its MFMAs: the registers of a second block would halve the
+    // workgroups that fit on a CU (measured slower).
+    constexpr bool PIPE = sizeof(R) == 8;
+    T xa[NT2][4], xb[NT2][4];
+    int64_t col = blockIdx.x;
+    if (PIPE && col < ncols && wave < nblk)
+        block_load<R, D2>(in + col * (int64_t)D + (int64_t)wave * BLK, lane, xa);
+    for (; col < ncols; col += gridDim.x) {
         const T *xc = in + col * (int64_t)D;
         T *oc = out + col * (int64_t)D;
+        const int64_t ncol = col + gridDim.x;
         V d2r[NT2], d2i[NT2];
         if constexpr (D3 == 0) {
             // no site-3 factor: every block goes from registers to HBM
             for (int blk = wave; blk < nblk; blk += WPB) {
-                block_modes12<R, D2, LDA2>(xc + (int64_t)blk * BLK, a1f, A2s, lane, d2r, d2i);
+                if (!PIPE) block_load<R, D2>(xc + (int64_t)blk * BLK, lane, xa);
+                block_modes12<R, D2, LDA2>(xa, a1f, A2s, lane, d2r, d2i);
+                if (PIPE && blk + WPB < nblk) block_load<R, D2>(xc + (int64_t)(blk + WPB) * BLK, lane, xa);
+                else if (PIPE && ncol < ncols)
+                    block_load<R, D2>(in + ncol * (int64_t)D + (int64_t)wave * BLK, lane, xa);
 #pragma unroll
                 for (int u = 0; u < NT2; ++u)
 #pragma unroll
@@ -142,11 +163,18 @@ __global__ void __launch_bounds__(REG_NT) k_lt_reg(const cplx<R> *__restrict__ A
             // the wave's first block is formed while other waves may still be
             // reading the previous column's planes; it is written after the
             // barrier, the wave's other blocks straight after they are formed
-            const bool first = wave < nblk;
-            if (first) block_modes12<R, D2, LDA2>(xc + (int64_t)wave * BLK, a1f, A2s, lane, d2r, d2i);
+            if (wave < nblk) {
+                if (!PIPE) block_load<R, D2>(xc + (int64_t)wave * BLK, lane, xa);
+                block_modes12<R, D2, LDA2>(xa, a1f, A2s, lane, d2r, d2i);
+                if (PIPE && wave + WPB < nblk) block_load<R, D2>(xc + (int64_t)(wave + WPB) * BLK, lane, xb);
+            }
             __syncthreads();  // every wave is done reading the previous column's planes
             for (int blk = wave; blk < nblk; blk += WPB) {
-                if (blk != wave) block_modes12<R, D2, LDA2>(xc + (int64_t)blk * BLK, a1f, A2s, lane, d2r, d2i);
+                if (blk != wave) {
+                    if (!PIPE) block_load<R, D2>(xc + (int64_t)blk * BLK, lane, xb);
+                    block_modes12<R, D2, LDA2>(xb, a1f, A2s, lane, d2r, d2i);
+                    if (PIPE && blk + WPB < nblk) block_load<R, D2>(xc + (int64_t)(blk + WPB) * BLK, lane, xb);
+                }
 #pragma unroll
                 for (int u = 0; u < NT2; ++u)
 #pragma unroll
@@ -157,6 +185,8 @@ __global__ void __launch_bounds__(REG_NT) k_lt_reg(const cplx<R> *__restrict__ A
                     }
             }
             __syncthreads();  // the column's planes are complete
+            if (PIPE && ncol < ncols && wave < nblk)
+                block_load<R, D2>(in + ncol * (int64_t)D + (int64_t)wave * BLK, lane, xa);
             // mode 3: Z^T (k3' x e) = A3 Y^T over groups of 16 elements e
             for (int g = wave; g < BLK / 16; g += WPB) {
                 const int e0 = 16 * g;
