@@ -595,7 +595,7 @@ def run_dense8192(args):
     mfma_flops = flops if reg or dt == np.complex128 else flops * 3 / 4
     roof = {"bound": "mfma", "kernel": "k_lt_reg" if reg else "k_lt_fused", "achieved": tflops, "peak": peak,
             "unit": "TFLOP/s",
-            "frac": tflops / peak, "traffic": traffic_of(f"lt_fused_{args.dtype}"),
+            "frac": tflops / peak, "traffic": traffic_of(f"{'lt_reg' if reg else 'lt_fused'}_{args.dtype}"),
             "algorithmic_flops_per_launch": flops, "mfma_flops_per_launch": mfma_flops,
             "executed_tflops": mfma_flops / (ms_fused * 1e-3) / 1e12,
             "executed_frac": mfma_flops / (ms_fused * 1e-3) / 1e12 / peak,

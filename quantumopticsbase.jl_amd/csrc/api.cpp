@@ -33,7 +33,7 @@ namespace qob {
 
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
-extern int g_qubit_nth, g_qubit_grid, g_qubit_remap;
+extern int g_qubit_nth, g_qubit_grid, g_qubit_remap, g_qubit_ntfar;
 extern int g_mode_kernel, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles, g_fused_resident;
 extern int g_fused_reg;
 extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel, g_csc_gremap, g_sell_cremap;
@@ -1348,7 +1348,13 @@ static bool compile_qubit(int nbits, const std::vector<TermSpec> &terms, QubitPr
         std::stable_partition(gs.begin(), gs.end(), [](const GroupOut &g) { return g.win.empty(); });
         int nconst = 0;
         for (auto &g : gs) nconst += g.win.empty() ? 1 : 0;
-        P.insert(P.end(), {nconst, (int32_t)gs.size() - nconst, 0, 0});
+        // header word 2: the first constant group whose flip leaves a tile of
+        // 2^min(tb, nbits) states (constant groups are in ascending F, so the
+        // ones before it read the tile and the rest read partner tiles)
+        int first_out = 0;
+        const int tbe = std::min(tb, nbits);
+        while (first_out < nconst && (gs[first_out].F >> tbe) == 0) ++first_out;
+        P.insert(P.end(), {nconst, (int32_t)gs.size() - nconst, pass == 0 ? first_out : 0, 0});
         for (auto &g : gs) {
             P.insert(P.end(), {(int32_t)g.F, (int32_t)g.win.size(), 0, 0});
             put_cd(P, g.c);
@@ -2760,6 +2766,7 @@ static int *tune_slot(const char *name) {
                {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
                {"qubit_split", &g_qubit_split},   {"mode_kernel", &g_mode_kernel}, {"fused_reg", &g_fused_reg},
+               {"qubit_ntfar", &g_qubit_ntfar},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
@@ -2794,10 +2801,11 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "fused_gpw") && value != 0 && value != 1 && value != 2 && value != 4)
         return fail(QOB_EARG, "fused_gpw must be 0 (automatic), 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
+    if (!strcmp(name, "qubit_remap") && value > 2) return fail(QOB_EARG, "qubit_remap must be 0, 1 or 2");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
          !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles") ||
-         !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg")) &&
+         !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg") || !strcmp(name, "qubit_ntfar")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
@@ -3154,6 +3162,7 @@ struct qob_batch_s {
     std::vector<int> nbits;
     std::vector<int32_t> prog_off, prog_hi_off;
     qob::DevBuf prog, vals, items, items_hi;
+    QubitSplit xs{}, xs_hi{};  // per-XCD ranges of items / items_hi
     qob::DevBuf slot[QOB_BATCH_SLOTS];
     bool bound[QOB_BATCH_SLOTS] = {false, false, false, false};
 };
@@ -3200,19 +3209,44 @@ qob_status qob_batch_create(qob_ctx ctx, int n, const qob_op *ops, qob_dtype dt,
         B->nbits.push_back(nb);
         B->total += (int64_t)1 << nb;
     }
-    // work items, largest states first
+    // Work items (state, tile) grouped by XCD.  A state's tiles read each
+    // other as partners of high-bit flips, so each state goes to one XCD whole
+    // (its partner tiles then sit in that XCD's L2); a state of more than
+    // 1/16 of the batch's tiles is cut into power-of-two runs of tiles first.
+    // Runs go largest first to the XCD with the fewest elements so far; each
+    // XCD walks its runs in that order and a run's tiles in index order.
     std::vector<int> order(n);
     for (int j = 0; j < n; ++j) order[j] = j;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->nbits[a] > B->nbits[b]; });
-    std::vector<QubitItemDev> items, items_hi;
+    int64_t ntiles_all = 0;
+    for (int j = 0; j < n; ++j) ntiles_all += (int64_t)1 << (B->nbits[j] - std::min(B->tbits, B->nbits[j]));
+    if (ntiles_all >= (1ll << 31)) return fail(QOB_EUNSUPPORTED, "batch too large");
+    int64_t run_cap = 1;
+    while (run_cap * 2 <= std::max<int64_t>(1, ntiles_all / 16)) run_cap *= 2;
+    std::vector<QubitItemDev> xl[8], xl_hi[8];
+    int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j : order) {
         const int tb = std::min(B->tbits, B->nbits[j]);
         const int64_t nt = (int64_t)1 << (B->nbits[j] - tb);
-        for (int64_t t = 0; t < nt; ++t) items.push_back({j, (int32_t)t});
-        if (B->prog_hi_off[j] >= 0)
-            for (int64_t t = 0; t < nt; ++t) items_hi.push_back({j, (int32_t)t});
+        const int64_t run = std::min(nt, run_cap);
+        for (int64_t t0 = 0; t0 < nt; t0 += run) {
+            const int x = (int)(std::min_element(load, load + 8) - load);
+            load[x] += run << tb;
+            for (int64_t t = t0; t < t0 + run; ++t) {
+                xl[x].push_back({j, (int32_t)t});
+                if (B->prog_hi_off[j] >= 0) xl_hi[x].push_back({j, (int32_t)t});
+            }
+        }
     }
-    if (items.size() >= (1ull << 31)) return fail(QOB_EUNSUPPORTED, "batch too large");
+    std::vector<QubitItemDev> items, items_hi;
+    for (int x = 0; x < 8; ++x) {
+        B->xs.off[x] = (int32_t)items.size();
+        B->xs_hi.off[x] = (int32_t)items_hi.size();
+        items.insert(items.end(), xl[x].begin(), xl[x].end());
+        items_hi.insert(items_hi.end(), xl_hi[x].begin(), xl_hi[x].end());
+    }
+    B->xs.off[8] = (int32_t)items.size();
+    B->xs_hi.off[8] = (int32_t)items_hi.size();
     B->nitems = (int)items.size();
     B->nitems_hi = (int)items_hi.size();
     if (!items_hi.empty()) QOB_TRY(B->items_hi.upload(items_hi.data(), items_hi.size() * sizeof(QubitItemDev)));
@@ -3253,11 +3287,11 @@ qob_status qob_batch_apply(qob_ctx ctx, qob_batch b, int slot, qob_c64 alpha, qo
     if (b->nitems_hi > 0)  // high-bit flips first: out = alpha * high + beta * out
         e = launch_qubit_hi(b->dt, (const QubitStateDev *)b->slot[slot].p, (const QubitItemDev *)b->items_hi.p,
                             b->nitems_hi, b->tbits, (const int32_t *)b->prog.p, b->vals.p, d2(alpha), d2(beta),
-                            strm(ctx, s));
+                            strm(ctx, s), &b->xs_hi);
     if (e == hipSuccess)
         e = launch_qubit(b->dt, (const QubitStateDev *)b->slot[slot].p, (const QubitItemDev *)b->items.p, b->nitems,
                          b->tbits, (const int32_t *)b->prog.p, b->vals.p, d2(alpha), d2(beta), d2(gamma),
-                         strm(ctx, s));
+                         strm(ctx, s), &b->xs);
     if (e != hipSuccess) return fail(QOB_EHIP, "batch apply: %s", hipGetErrorString(e));
     return QOB_OK;
 }

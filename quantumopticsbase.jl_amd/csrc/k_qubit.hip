@@ -46,6 +46,24 @@ __device__ __forceinline__ uint32_t qb_xcd_remap(uint32_t b, uint32_t G) {
     return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// The items one workgroup walks.  Per-XCD ranges (flags & 8): workgroup b
+// runs on XCD x = b % 8 and walks [off[x], off[x + 1]) with a stride of the
+// number of workgroups on that XCD (the grid has at least 8 workgroups then).
+// Otherwise rounds of G items over the whole list, each XCD a contiguous run
+// of every round when flags & 4.
+struct ItemWalk {
+    int it, end, step;
+};
+__device__ __forceinline__ ItemWalk qb_walk(int flags, const QubitSplit &xs, int nitems) {
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    if (flags & 8) {
+        const uint32_t x = b & 7u;
+        return {xs.off[x] + (int)(b >> 3), xs.off[x + 1], (int)((G - x + 7u) >> 3)};
+    }
+    const uint32_t lb = (flags & 4) ? qb_xcd_remap(b, G) : b;
+    return {(int)lb, nitems, (int)G};
+}
+
 template <typename T> __device__ __forceinline__ T to_t(double2 a) {
     T r; r.x = (typename real_of<T>::type)a.x; r.y = (typename real_of<T>::type)a.y; return r;
 }
@@ -57,24 +75,48 @@ template <typename T> __device__ __forceinline__ T rec_weight(const int32_t *r) 
     w.y = (typename real_of<T>::type)__hiloint2double(r[3], r[2]);
     return w;
 }
+// the same, held in scalar registers: a weight carried from one loop
+// iteration to the next is otherwise re-read with a vector load, whose wait
+// would also wait for every partner load issued before it
+template <typename T> __device__ __forceinline__ T rec_weight_s(const int32_t *r) {
+    T w;
+    w.x = (typename real_of<T>::type)__hiloint2double(__builtin_amdgcn_readfirstlane(r[1]),
+                                                      __builtin_amdgcn_readfirstlane(r[0]));
+    w.y = (typename real_of<T>::type)__hiloint2double(__builtin_amdgcn_readfirstlane(r[3]),
+                                                      __builtin_amdgcn_readfirstlane(r[2]));
+    return w;
+}
 
 // acc += w * x for every element of the lane, w uniform; a real weight
 // needs two FMAs instead of four.  Partner of element k (tile index
 // e = tid + k NTH) under flip F is e ^ F = (tid ^ F) ^ (k NTH): one XOR with a
 // compile-time constant per element, on the byte offset tpb = (tid ^ F) * sizeof(T).
-template <typename T, int NTH, int EPT>
+// NT: a partner tile read once (non-temporal load: it is not kept in L2 at
+// the expense of tiles read again soon)
+template <bool NT, typename T> __device__ __forceinline__ T qb_ld(const T *p) {
+    if constexpr (NT) return ldg_nt(p);
+    else return ldg(p);
+}
+
+// GL: basep points to global memory (a partner tile), else to the LDS tile
+template <typename T, int NTH, int EPT, bool GL = false, bool NT = false>
 __device__ __forceinline__ void qb_uniform(T wt, const char *basep, uint32_t tpb, T (&acc)[EPT]) {
+    auto rd = [&](int k) -> T {
+        const T *p = (const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+        if constexpr (GL) return qb_ld<NT>(p);
+        else return *p;
+    };
     if (wt.y == 0) {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+            const T x = rd(k);
             acc[k].x = fma(wt.x, x.x, acc[k].x);
             acc[k].y = fma(wt.x, x.y, acc[k].y);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const T x = *(const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
+            const T x = rd(k);
             acc[k] = cfma(wt, x, acc[k]);
         }
     }
@@ -110,12 +152,86 @@ template <int TBITS> struct HiMap {
 // partial tile never leave it (the state is a single tile then).
 template <typename T, int NTH, int EPT, bool LO, class Map>
 __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *__restrict__ vals, const T *tile,
-                                       const T *__restrict__ in, const Map &m, T (&acc)[EPT]) {
+                                       const T *__restrict__ in, const Map &m, T (&acc)[EPT],
+                                       uint32_t far_tiles = 0xffffffffu) {
     const uint32_t tid = threadIdx.x;
     const int nconst = pg[0];
     const int nwg = pg[1];
     const int32_t *r = pg + 4;
-    if (nconst > 0) {
+    if constexpr (LO) {
+        // Partner values of group gr into dst, from global memory: the tile
+        // of basis states (base + e) ^ F = (base ^ Fhigh) + (e ^ Flow); also
+        // right (if slower than LDS) for a flip inside the tile.
+        auto issue = [&](uint32_t Fp, T (&dst)[EPT]) {
+            const char *gp = (const char *)(in + (m.J(0) ^ (Fp & ~m.lowmask)));
+            const uint32_t tpb = (tid ^ (Fp & m.lowmask)) * (uint32_t)sizeof(T);
+            if ((Fp >> m.tb) >= far_tiles) {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k)
+                    dst[k] = ldg_nt((const T *)(gp + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T)))));
+            } else {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k)
+                    dst[k] = ldg((const T *)(gp + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T)))));
+            }
+        };
+        auto accum = [&](T wt, const T (&x)[EPT]) {
+            if (wt.y == 0) {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    acc[k].x = fma(wt.x, x[k].x, acc[k].x);
+                    acc[k].y = fma(wt.x, x[k].y, acc[k].y);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) acc[k] = cfma(wt, x[k], acc[k]);
+            }
+        };
+        // Constant groups are in ascending F: groups [0, nin) flip bits inside
+        // the tile (header word 2), the rest read partner tiles.  Partner
+        // loads run one group ahead through two buffers, the first group's
+        // during the in-tile groups; accumulation stays in program order.
+        // (nin is computed on the host for this tile size; the tile mask keeps
+        // every LDS read inside the tile whatever the program holds)
+        const int nin = min(max(pg[2], 0), nconst);
+        T A[EPT], B[EPT];
+        // the first out-of-tile group's loads fly during the in-tile groups
+        if (nin < nconst) issue((uint32_t)__builtin_amdgcn_readfirstlane(r[8 * nin]), A);
+        if (nin > 0) {
+            uint32_t F = (uint32_t)__builtin_amdgcn_readfirstlane(r[0]);
+            T wt = rec_weight_s<T>(r + 4);
+            for (int g = 0; g < nin; ++g) {
+                const int32_t *rn = r + 8 * (g + 1 < nin ? g + 1 : g);
+                const uint32_t Fn = (uint32_t)__builtin_amdgcn_readfirstlane(rn[0]);
+                const T wn = rec_weight_s<T>(rn + 4);
+                qb_uniform<T, NTH, EPT>(wt, (const char *)tile,
+                                        ((tid ^ m.fq(F)) & m.lowmask) * (uint32_t)sizeof(T), acc);
+                F = Fn;
+                wt = wn;
+            }
+        }
+        if (nin < nconst) {
+            // Two buffers in a loop over pairs of groups, every group's loads
+            // issued unconditionally (the last one's once more when the count
+            // is even), so that each wait counts the same loads on every
+            // path: A holds group g while B fills with group g + 1.
+            int g = nin;
+            const int last = nconst - 1;
+            const int npairs = (nconst - nin) >> 1;
+            for (int pq = 0; pq < npairs; ++pq) {
+                const uint32_t F1 = (uint32_t)__builtin_amdgcn_readfirstlane(r[8 * (g + 1)]);
+                const uint32_t F2 = (uint32_t)__builtin_amdgcn_readfirstlane(r[8 * min(g + 2, last)]);
+                const T w1 = rec_weight_s<T>(r + 8 * g + 4);
+                const T w2 = rec_weight_s<T>(r + 8 * (g + 1) + 4);
+                issue(F1, B);
+                accum(w1, A);
+                issue(F2, A);
+                accum(w2, B);
+                g += 2;
+            }
+            if (g < nconst) accum(rec_weight_s<T>(r + 8 * g + 4), A);
+        }
+    } else if (nconst > 0) {
         uint32_t F = (uint32_t)r[0];
         T wt = rec_weight<T>(r + 4);
         for (int g = 0; g < nconst; ++g) {
@@ -124,13 +240,9 @@ __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *
             const T wn = rec_weight<T>(rn + 4);
             if (m.in_tile(F)) {
                 qb_uniform<T, NTH, EPT>(wt, (const char *)tile, (tid ^ m.fq(F)) * (uint32_t)sizeof(T), acc);
-            } else if (LO) {
-                // partner tile in global memory: (base + e) ^ F = (base ^ Fhigh) + (e ^ Flow)
-                const char *gp = (const char *)(in + (m.J(0) ^ (F & ~m.lowmask)));
-                qb_uniform<T, NTH, EPT>(wt, gp, (tid ^ (F & m.lowmask)) * (uint32_t)sizeof(T), acc);
             } else {
 #pragma unroll
-                for (int k = 0; k < EPT; ++k) acc[k] = cfma(wt, in[m.J(tid + k * NTH) ^ F], acc[k]);
+                for (int k = 0; k < EPT; ++k) acc[k] = cfma(wt, ldg(in + (m.J(tid + k * NTH) ^ F)), acc[k]);
             }
             F = Fn;
             wt = wn;
@@ -148,9 +260,23 @@ __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *
         for (int q = 0; q < nwin; ++q, r += 4) {
             const int sh = r[0], toff = r[2];
             const uint32_t mask = (1u << r[1]) - 1u;
+            bool above = false;
+            if constexpr (LO) above = sh >= m.tb;
+            if (above) {
+                // window above the tile: one entry for the whole tile
+                const T v = vals[toff + (int)((m.J(0) >> sh) & mask)];
 #pragma unroll
-            for (int k = 0; k < EPT; ++k)
-                wt[k] = cadd(wt[k], vals[toff + (int)((m.J(tid + k * NTH) >> sh) & mask)]);
+                for (int k = 0; k < EPT; ++k) wt[k] = cadd(wt[k], v);
+            } else if (LO && (NTH >> sh) > (int)mask) {
+                // window inside the lane index bits: one entry per lane
+                const T v = vals[toff + (int)((tid >> sh) & mask)];
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) wt[k] = cadd(wt[k], v);
+            } else {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k)
+                    wt[k] = cadd(wt[k], vals[toff + (int)((m.J(tid + k * NTH) >> sh) & mask)]);
+            }
         }
         // a zero weight must not propagate a NaN / Inf partner (rows absent
         // from the factors): select instead of multiply
@@ -167,7 +293,7 @@ __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *
         } else {
 #pragma unroll
             for (int k = 0; k < EPT; ++k) {
-                const T x = in[m.J(tid + k * NTH) ^ F];
+                const T x = ldg(in + (m.J(tid + k * NTH) ^ F));
                 const T v = cfma(wt[k], x, acc[k]);
                 const bool nz = wt[k].x != 0 || wt[k].y != 0;
                 acc[k].x = nz ? v.x : acc[k].x;
@@ -185,19 +311,22 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
                                                const QubitItemDev *__restrict__ items, int nitems,
                                                const int32_t *__restrict__ prog, const T *__restrict__ vals,
                                                double2 alpha_, double2 beta_, double2 gamma_, int flags,
-                                               QubitStateDev single, int ntiles1) {
+                                               QubitStateDev single, int ntiles1, QubitSplit xs) {
     constexpr int EPT = (1 << TBITS) / NTH;
-    const uint32_t G = gridDim.x;
-    const uint32_t lb = (flags & 4) ? qb_xcd_remap(blockIdx.x, G) : blockIdx.x;
     extern __shared__ unsigned char lds_raw[];
     T *tile = (T *)lds_raw;
     const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_), gamma = to_t<T>(gamma_);
     const bool beta_zero = flags & 1, gamma_zero = flags & 2;
     const int tid = threadIdx.x;
-    // items in rounds of G: all workgroups work on neighbouring tiles at the
-    // same time (partners of high-bit flips are being read concurrently), and
-    // within a round each XCD owns a contiguous run of tiles
-    for (int it = (int)lb; it < nitems; it += (int)G) {
+    // per-XCD ranges keep the partner tiles of a state's high-bit flips in the
+    // L2 of the XCD that reads them; rounds of G keep all workgroups on
+    // neighbouring tiles at the same time
+    const ItemWalk wk = qb_walk(flags, xs, nitems);
+    // with flags & 16, partner tiles at least as many tiles away as an XCD
+    // has workgroups lie outside the run of tiles in flight there: read once
+    const uint32_t far_tiles =
+        !(flags & 16) ? 0xffffffffu : (flags & 8) ? (uint32_t)wk.step : max(1u, (uint32_t)gridDim.x >> 3);
+    for (int it = wk.it; it < wk.end; it += wk.step) {
         QubitItemDev w;
         QubitStateDev S;
         if (states) {
@@ -225,14 +354,14 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
-            if (e < tsize) tile[e] = in[base + e];
+            if (e < tsize) tile[e] = ldg(in + base + e);
         }
         __syncthreads();
         T acc[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
         const LoMap m{base, (uint32_t)tsize - 1u, tb};
-        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, in, m, acc);
+        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, in, m, acc, far_tiles);
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
@@ -240,8 +369,8 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
                 const uint32_t J = base + (uint32_t)e;
                 T r = cmul(alpha, acc[k]);
                 if (!gamma_zero) r = cfma(gamma, tile[e], r);
-                if (!bz) r = cfma(beff, out[J], r);
-                st_stream(out + J, r);
+                if (!bz) r = cfma(beff, ldg(out + J), r);
+                stg_stream(out + J, r);
             }
         }
     }
@@ -258,16 +387,15 @@ __global__ void __launch_bounds__(NTH) k_qubit_hi(const QubitStateDev *__restric
                                                   const QubitItemDev *__restrict__ items, int nitems,
                                                   const int32_t *__restrict__ prog, const T *__restrict__ vals,
                                                   double2 alpha_, double2 beta_, int flags, QubitStateDev single,
-                                                  int ntiles1) {
+                                                  int ntiles1, QubitSplit xs) {
     constexpr int EPT = (1 << TBITS) / NTH;
     extern __shared__ unsigned char lds_raw[];
     T *tile = (T *)lds_raw;
     const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
     const bool beta_zero = flags & 1;
     const int tid = threadIdx.x;
-    const uint32_t G = gridDim.x;
-    const uint32_t lb = (flags & 4) ? qb_xcd_remap(blockIdx.x, G) : blockIdx.x;
-    for (int it = (int)lb; it < nitems; it += (int)G) {
+    const ItemWalk wk = qb_walk(flags, xs, nitems);
+    for (int it = wk.it; it < wk.end; it += wk.step) {
         QubitItemDev w;
         QubitStateDev S;
         if (states) {
@@ -290,7 +418,7 @@ __global__ void __launch_bounds__(NTH) k_qubit_hi(const QubitStateDev *__restric
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const uint32_t q = (uint32_t)(tid + k * NTH);
-            tile[q] = in[m.J(q)];
+            tile[q] = ldg(in + m.J(q));
         }
         __syncthreads();
         T acc[EPT];
@@ -301,8 +429,8 @@ __global__ void __launch_bounds__(NTH) k_qubit_hi(const QubitStateDev *__restric
         for (int k = 0; k < EPT; ++k) {
             const uint32_t J = m.J((uint32_t)(tid + k * NTH));
             T r = cmul(alpha, acc[k]);
-            if (!beta_zero) r = cfma(beta, out[J], r);
-            out[J] = r;
+            if (!beta_zero) r = cfma(beta, ldg(out + J), r);
+            stg(out + J, r);
         }
     }
 }
@@ -330,7 +458,7 @@ __global__ void __launch_bounds__(NTH) k_qubit_dot(QubitStateDev S, const T *__r
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
-            if (e < tsize) tile[e] = v[base + e];
+            if (e < tsize) tile[e] = ldg(v + base + e);
         }
         __syncthreads();
         T acc[EPT];
@@ -379,26 +507,48 @@ __global__ void __launch_bounds__(NTH) k_qubit_axpy(const QubitStateDev *__restr
         const uint32_t base = (uint32_t)w.tile << tb;
         const T *__restrict__ in = (const T *)S.in;
         T *__restrict__ out = (T *)S.out;
-        for (int e = threadIdx.x; e < (1 << tb); e += NTH) out[base + e] = cfma(a, in[base + e], out[base + e]);
+        for (int e = threadIdx.x; e < (1 << tb); e += NTH)
+            stg(out + base + e, cfma(a, ldg(in + base + e), ldg(out + base + e)));
     }
 }
 
 }  // namespace
 
 int g_qubit_nth = 512;
-int g_qubit_remap = 1;
+// 0: workgroup b walks items b, b + G, ...; 1: the same rounds of G items with
+// each XCD given a contiguous run of every round; 2: each XCD walks its own
+// item range for the whole launch (a batch's states are assigned to XCDs
+// whole; the columns form splits its items into 8 contiguous ranges).
+int g_qubit_remap = 2;
+// 1: partner tiles of flips whose tile distance is at least the number of
+// workgroups walking the XCD's range are read with non-temporal loads
+int g_qubit_ntfar = 1;
+
+// flags bits 4 / 8 and the item ranges of a launch with g workgroups
+static int qubit_walk_flags(int nitems, unsigned g, const QubitSplit *xs, QubitSplit &use) {
+    if (g_qubit_remap == 2 && g >= 8) {
+        if (xs) {
+            use = *xs;
+        } else {
+            for (int x = 0; x <= 8; ++x) use.off[x] = (int32_t)((int64_t)nitems * x / 8);
+        }
+        return 8;
+    }
+    for (int x = 0; x <= 8; ++x) use.off[x] = 0;
+    return g_qubit_remap ? 4 : 0;
+}
 // 0: one workgroup per item when the items fit in 5 workgroups per CU (many small states: no
-// persistent loop, every item starts at once); otherwise a persistent grid of 3 workgroups per CU
-// for ComplexF64 and 4 for ComplexF32. With 2^11-element tiles that keeps about 2-3 MB of tiles in
-// flight per XCD, inside its 4 MB L2, so partner tiles read by high-bit flips are still resident;
-// 5 per CU measured 5-15% slower on a batch of 256 chains of 8..20 sites.
+// persistent loop, every item starts at once); otherwise a persistent grid of 2 workgroups per CU
+// for ComplexF64 and 4 for ComplexF32, the most that fit by registers (84 and 54 VGPRs at 512
+// threads). Each wave keeps two flip groups' partner loads in flight; 2^11-element tiles keep
+// 2-3 MB of tiles in flight per XCD, inside its 4 MB L2.
 int g_qubit_grid = 0;
 
 static unsigned qubit_grid(qob_dtype dt, int nitems) {
     const int ncu = device_cu_count();
     const int64_t g = g_qubit_grid > 0           ? g_qubit_grid
                       : nitems <= (int64_t)ncu * 5 ? nitems
-                                                   : (int64_t)ncu * (dt == QOB_C64 ? 3 : 4);
+                                                   : (int64_t)ncu * (dt == QOB_C64 ? 2 : 4);
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, nitems));
 }
 
@@ -406,12 +556,14 @@ int g_qubit_tbits = 11;
 
 static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
                                    int tbits, const int32_t *prog, const void *vals, double2 alpha, double2 beta,
-                                   double2 gamma, hipStream_t s, QubitStateDev single, int ntiles1);
+                                   double2 gamma, hipStream_t s, QubitStateDev single, int ntiles1,
+                                   const QubitSplit *xs);
 
 hipError_t launch_qubit(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
                         const int32_t *prog, const void *vals, double2 alpha, double2 beta, double2 gamma,
-                        hipStream_t s) {
-    return launch_qubit_any(dt, states, items, nitems, tbits, prog, vals, alpha, beta, gamma, s, QubitStateDev(), 1);
+                        hipStream_t s, const QubitSplit *xs) {
+    return launch_qubit_any(dt, states, items, nitems, tbits, prog, vals, alpha, beta, gamma, s, QubitStateDev(), 1,
+                            xs);
 }
 
 hipError_t launch_qubit_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
@@ -421,20 +573,22 @@ hipError_t launch_qubit_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, 
     const int64_t nitems = ntiles * ncols;
     if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
     return launch_qubit_any(dt, nullptr, nullptr, (int)nitems, tbits, prog, vals, alpha, beta, gamma, s, single,
-                            (int)ntiles);
+                            (int)ntiles, nullptr);
 }
 
 static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
                                    int tbits, const int32_t *prog, const void *vals, double2 alpha, double2 beta,
-                                   double2 gamma, hipStream_t s, QubitStateDev single, int ntiles1) {
+                                   double2 gamma, hipStream_t s, QubitStateDev single, int ntiles1,
+                                   const QubitSplit *xs) {
     if (nitems == 0) return hipSuccess;
-    const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (gamma.x == 0.0 && gamma.y == 0.0 ? 2 : 0) |
-                      (g_qubit_remap ? 4 : 0);
     if (tbits < 10) tbits = 10;
     if (tbits > 13) tbits = 13;
     const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const unsigned g = qubit_grid(dt, nitems);
+    QubitSplit ws;
+    const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (gamma.x == 0.0 && gamma.y == 0.0 ? 2 : 0) |
+                      qubit_walk_flags(nitems, g, xs, ws) | (g_qubit_ntfar ? 16 : 0);
 #define QOB_QB(TT, NT, TB)                                                                                    \
     do {                                                                                                      \
         auto kfn = k_qubit<TT, NT, TB>;                                                                       \
@@ -446,7 +600,7 @@ static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, co
             lds_set = 160 * 1024;                                                                             \
         }                                                                                                     \
         hipLaunchKernelGGL(kfn, dim3(g), dim3(NT), lds, s, states, items, nitems, prog, (const TT *)vals,     \
-                           alpha, beta, gamma, flags, single, ntiles1);                                      \
+                           alpha, beta, gamma, flags, single, ntiles1, ws);                                  \
         return hipGetLastError();                                                                             \
     } while (0)
 #define QOB_QB_NT(TT, TB)                                  \
@@ -472,12 +626,14 @@ static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, co
 
 static hipError_t launch_qubit_hi_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items,
                                       int nitems, int tbits, const int32_t *prog, const void *vals, double2 alpha,
-                                      double2 beta, hipStream_t s, QubitStateDev single, int ntiles1) {
+                                      double2 beta, hipStream_t s, QubitStateDev single, int ntiles1,
+                                      const QubitSplit *xs) {
     if (nitems == 0) return hipSuccess;
     if (tbits < 10 || tbits > 13) return hipErrorInvalidValue;
-    const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (g_qubit_remap ? 4 : 0);
     const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
     const unsigned g = qubit_grid(dt, nitems);
+    QubitSplit ws;
+    const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | qubit_walk_flags(nitems, g, xs, ws);
 #define QOB_QH(TT, NT, TB)                                                                                    \
     do {                                                                                                      \
         auto kfn = k_qubit_hi<TT, NT, TB>;                                                                    \
@@ -489,7 +645,7 @@ static hipError_t launch_qubit_hi_any(qob_dtype dt, const QubitStateDev *states,
             lds_set = 160 * 1024;                                                                             \
         }                                                                                                     \
         hipLaunchKernelGGL(kfn, dim3(g), dim3(NT), lds, s, states, items, nitems, prog, (const TT *)vals,     \
-                           alpha, beta, flags, single, ntiles1);                                              \
+                           alpha, beta, flags, single, ntiles1, ws);                                          \
         return hipGetLastError();                                                                             \
     } while (0)
 #define QOB_QH_NT(TT, TB)                                  \
@@ -513,8 +669,9 @@ static hipError_t launch_qubit_hi_any(qob_dtype dt, const QubitStateDev *states,
 }
 
 hipError_t launch_qubit_hi(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
-                           const int32_t *prog, const void *vals, double2 alpha, double2 beta, hipStream_t s) {
-    return launch_qubit_hi_any(dt, states, items, nitems, tbits, prog, vals, alpha, beta, s, QubitStateDev(), 1);
+                           const int32_t *prog, const void *vals, double2 alpha, double2 beta, hipStream_t s,
+                           const QubitSplit *xs) {
+    return launch_qubit_hi_any(dt, states, items, nitems, tbits, prog, vals, alpha, beta, s, QubitStateDev(), 1, xs);
 }
 
 hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
@@ -524,7 +681,7 @@ hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncol
     const int64_t nitems = ntiles * ncols;
     if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
     return launch_qubit_hi_any(dt, nullptr, nullptr, (int)nitems, tbits, prog, vals, alpha, beta, s, single,
-                               (int)ntiles);
+                               (int)ntiles, nullptr);
 }
 
 int qubit_dot_blocks(int64_t ntiles) {

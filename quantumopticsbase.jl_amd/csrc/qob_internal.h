@@ -63,6 +63,47 @@ template <typename T> __device__ __forceinline__ void st_out(T *p, T v, bool nt)
     else *p = v;
 }
 
+// Accesses through pointers that a kernel reads from a device table (or a
+// struct argument): the compiler cannot tell that they point to global
+// memory and would emit flat instructions, which also count against the
+// LDS wait counter, so that every LDS wait waits for them as well.  These
+// name the global address space.
+#define QOB_GLOBAL __attribute__((address_space(1)))
+template <typename T> struct gvec;
+template <> struct gvec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct gvec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <typename T> __device__ __forceinline__ T ldg(const T *p) {
+    using V = typename gvec<T>::type;
+    const V v = *(const QOB_GLOBAL V *)p;
+    T r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+// non-temporal load (data read once by this launch)
+template <typename T> __device__ __forceinline__ T ldg_nt(const T *p) {
+    using V = typename gvec<T>::type;
+    const V v = __builtin_nontemporal_load((const QOB_GLOBAL V *)p);
+    T r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+template <typename T> __device__ __forceinline__ void stg(T *p, T v) {
+    using V = typename gvec<T>::type;
+    V w;
+    w.x = v.x;
+    w.y = v.y;
+    *(QOB_GLOBAL V *)p = w;
+}
+template <typename T> __device__ __forceinline__ void stg_stream(T *p, T v) {
+    using V = typename gvec<T>::type;
+    V w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, (QOB_GLOBAL V *)p);
+}
+
 // ------------------------------------------------------ fast integer divide
 // q = n / d for 0 <= n < 2^31 with a host-computed multiplier:
 //   l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
@@ -250,10 +291,17 @@ struct QubitItemDev {
     int32_t state;
     int32_t tile;
 };
-// Batched form: items (state, tile) index a device table of states.
+// Items split among the 8 XCDs of the device: the workgroups b with
+// b % 8 == x (the ones the dispatcher places on XCD x) walk the items
+// [off[x], off[x + 1]), so all tiles of one state share one L2.
+struct QubitSplit {
+    int32_t off[9];
+};
+// Batched form: items (state, tile) index a device table of states.  xs:
+// per-XCD item ranges (the items grouped by XCD), or null.
 hipError_t launch_qubit(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
                         const int32_t *prog, const void *vals, double2 alpha, double2 beta, double2 gamma,
-                        hipStream_t s);
+                        hipStream_t s, const QubitSplit *xs = nullptr);
 // Columns form: ncols states of one program, state c at in/out + c * 2^nbits.
 hipError_t launch_qubit_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
                              const void *vals, double2 alpha, double2 beta, double2 gamma, hipStream_t s);
@@ -262,7 +310,8 @@ hipError_t launch_qubit_axpy(qob_dtype dt, const QubitStateDev *states, const Qu
 // Second pass over strided tiles (high-bit flips) for states with prog_hi_off >= 0:
 // out = alpha * high + beta * out; the first pass then adds to out.
 hipError_t launch_qubit_hi(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems, int tbits,
-                           const int32_t *prog, const void *vals, double2 alpha, double2 beta, hipStream_t s);
+                           const int32_t *prog, const void *vals, double2 alpha, double2 beta, hipStream_t s,
+                           const QubitSplit *xs = nullptr);
 hipError_t launch_qubit_hi_cols(qob_dtype dt, QubitStateDev single, int64_t ncols, int tbits, const int32_t *prog,
                                 const void *vals, double2 alpha, double2 beta, hipStream_t s);
 // out (D x ncols) = alpha * (A_n ... A_1) in + beta * out
