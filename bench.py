@@ -18,6 +18,8 @@ Other workloads (--workload):
              mul!(out, H, psi, 1, 0) on a normalised complex-Gaussian Ket
   batch256   256 transverse-field Ising chains of 8..20 sites, one launch per
              step x' = x + c H x for all of them
+  denseop    DenseOperator (8192 x 8192) on a density matrix (MFMA GEMM), on a
+             Ket and on a Bra (gemv)
   serve      Poisson request stream of mixed jobs (32 x (mul! + axpy) +
              normalize!) over the four shapes above; p50/p99 latency
 
@@ -622,6 +624,72 @@ def run_dense8192(args):
     return res
 
 
+# ---------------------------------------------------------------- denseop
+def run_denseop(args):
+    """DenseOperator mul! (src/operators_dense.jl:394-396) on an 8192 x 8192
+    density matrix (k_zgemm), on a Ket (k_zgemv) and on a Bra (k_zgemv_t)."""
+    import qobhip as Q
+    from qobhip import workloads as W
+    from qobhip.tune import EventTimer
+
+    dt = np.complex64 if args.dtype == "c32" else np.complex128
+    N = 8192
+    g = W.rng(11)
+    b = Q.GenericBasis(N)
+    A_h = W.complex_gaussian(g, (N, N)).astype(dt)
+    A = Q.Operator(b, b, A_h)
+    X = Q.Operator(b, b, W.complex_gaussian(g, (N, N)).astype(dt))
+    Y = Q.Operator(b, b, Q.DeviceArray(N * N, dt, (N, N)))
+    k = Q.Ket(b, Q.DeviceArray.from_numpy(W.complex_gaussian(g, (N,)).astype(dt), dt))
+    ko = Q.Ket(b, Q.DeviceArray(N, dt))
+    br = Q.Bra(b, Q.DeviceArray.from_numpy(W.complex_gaussian(g, (N,)).astype(dt), dt))
+    bo = Q.Bra(b, Q.DeviceArray(N, dt))
+    ctx = Q.context()
+    rho_op = lambda: Q.mul_(Y, A, X, 1.0, 0.0)
+    ket_op = lambda: Q.mul_(ko, A, k, 1.0, 0.0)
+    bra_op = lambda: Q.mul_(bo, br, A, 1.0, 0.0)
+    if args.probe:
+        return probe(ctx, X.data, Y.data, rho_op)
+    steps = max(3, min(args.steps, 20))
+    log(f"bench: denseop {args.dtype} warmup {args.warmup}, timing {steps} steps")
+    el, ev_ms = timed_events(ctx, rho_op, steps, max(1, min(args.warmup, 3)))
+    ms_gemm = ev_ms / steps
+    ev = EventTimer(ctx.h, ctx.stream)
+    ket_op()
+    bra_op()
+    ctx.synchronize()
+    ms_gemv = ev(ket_op, 50)
+    ms_gemv_t = ev(bra_op, 50)
+    esz = np.dtype(dt).itemsize
+    flops = 8.0 * N ** 3
+    peak = FP32_MFMA_TFLOPS if dt == np.complex64 else FP64_MFMA_TFLOPS
+    tflops = flops / (ms_gemm * 1e-3) / 1e12
+    gemv_bytes = (N * N + 2 * N) * esz
+    cpu = None
+    if not args.no_cpu:
+        nthr = blas_threads()
+        log(f"bench: CPU baseline (numpy BLAS, {nthr} threads)")
+        X_h = X.to_numpy()
+        n, t0 = 0, time.perf_counter()
+        while True:
+            _ = A_h @ X_h
+            n += 1
+            t = time.perf_counter() - t0
+            if t > 10.0:
+                break
+        cpu = {"value": n / t, "unit": "applies/s", "cores": nthr, "kind": "port",
+               "sample": f"{n} products A rho of the same 8192x8192 operands, numpy BLAS (the reference's "
+                         f"mul! calls BLAS gemm, src/operators_dense.jl:394), {nthr} threads, {t:.1f} s"}
+    roof = {"bound": "mfma", "kernel": "k_zgemm", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
+            "frac": tflops / peak, "traffic": None, "algorithmic_flops_per_launch": flops,
+            "kernel_ms_avg": ms_gemm, "timing": "events on the library stream around the timed steps"}
+    config = {"workload": f"DenseOperator mul! on an 8192x8192 {args.dtype} density matrix (left); 1 step = 1 apply",
+              "N": N, "kernel_ms": {"zgemm_rho": ms_gemm, "zgemv_ket": ms_gemv, "zgemv_t_bra": ms_gemv_t},
+              "gemv_gbs": {"ket": gemv_bytes / (ms_gemv * 1e-3) / 1e9, "bra": gemv_bytes / (ms_gemv_t * 1e-3) / 1e9}}
+    return base_result(args, steps / el, "applies/s", el * 1e3 / steps, "f32" if dt == np.complex64 else "f64",
+                       "synthetic (complex Gaussian operator and matrices)", config, roof, cpu)
+
+
 # ---------------------------------------------------------------- batch256
 def run_batch256(args):
     import qobhip as Q
@@ -764,9 +832,9 @@ def run_serve(args):
 
 
 WORKLOADS = {"sparse6400": run_sparse6400, "ket10": run_ket10, "dense8192": run_dense8192,
-             "batch256": run_batch256, "serve": run_serve}
+             "batch256": run_batch256, "denseop": run_denseop, "serve": run_serve}
 DEFAULT_STEPS = {"sparse6400": (200, 10), "ket10": (2000, 100), "dense8192": (100, 20), "batch256": (200, 10),
-                 "serve": (0, 0)}
+                 "denseop": (10, 2), "serve": (0, 0)}
 
 
 def main():

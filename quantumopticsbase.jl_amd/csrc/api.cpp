@@ -239,23 +239,35 @@ int g_sell_align = 1;
 // components are the blocks of a conserved quantity (the chains of a
 // Jaynes-Cummings-like exchange term, for example).  Returns false
 // (contiguous slices) for any other X.
+//
+// Line-aligned form (ComplexF64, sell_cline = 1): rows are also joined
+// within aligned groups of 8 (the 128-byte line of a column of R), so every
+// run of rows in a block is whole lines and no two blocks write into one
+// line.  The components then grow; they are packed into blocks of 320 rows
+// (5 slices), used when that fills the blocks at least 90%.
 int g_sell_closed = 1;
+int g_sell_cline = 1;
 
-static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
-    constexpr int BLK = 256;
+// lanes (block-major, BLK per block, -1 idle) of the closed layout with rows
+// joined in aligned groups of grp; false if a component exceeds BLK rows
+static bool closed_lanes(const RowLists &X, int grp, int BLK, std::vector<int32_t> &lanes) {
     const int64_t m = X.nr;
-    if (m != X.nc || m < 2 || m >= (1ll << 31)) return false;
     std::vector<int32_t> parent((size_t)m);
     for (int64_t i = 0; i < m; ++i) parent[i] = (int32_t)i;
     auto find = [&](int32_t a) {
         while (parent[a] != a) a = parent[a] = parent[parent[a]];
         return a;
     };
+    auto unite = [&](int32_t a, int32_t b) {
+        a = find(a);
+        b = find(b);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    };
     for (int64_t i = 0; i < m; ++i)
-        for (int32_t q = X.ptr[i]; q < X.ptr[i + 1]; ++q) {
-            int32_t a = find((int32_t)i), b = find(X.idx[q]);
-            if (a != b) parent[std::max(a, b)] = std::min(a, b);
-        }
+        for (int32_t q = X.ptr[i]; q < X.ptr[i + 1]; ++q) unite((int32_t)i, X.idx[q]);
+    if (grp > 1)
+        for (int64_t i = 0; i < m; ++i)
+            if (i % grp) unite((int32_t)i, (int32_t)(i - i % grp));
     std::vector<int32_t> size((size_t)m, 0);
     for (int64_t i = 0; i < m; ++i) {
         if (++size[find((int32_t)i)] > BLK) return false;
@@ -266,7 +278,7 @@ static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
     for (int64_t i = 0; i < m; ++i) start[i + 1] += start[i];
     cur.assign(start.begin(), start.end() - 1);
     for (int64_t i = 0; i < m; ++i) rows_of[cur[find((int32_t)i)]++] = (int32_t)i;
-    std::vector<int32_t> lanes;  // block-major, BLK per block, -1 idle
+    lanes.clear();
     int used = BLK;
     for (int64_t r = 0; r < m; ++r) {
         if (start[r + 1] == start[r]) continue;  // r is not a component root
@@ -279,11 +291,25 @@ static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
         for (int k = 0; k < sz; ++k) lanes[base + used + k] = rows_of[start[r] + k];
         used += sz;
     }
-    const int nblocks = (int)(lanes.size() / BLK), nslices = nblocks * (BLK / 64);
+    const int nblocks = (int)(lanes.size() / BLK);
     for (int k = 0; k < nblocks; ++k) {
         auto b = lanes.begin() + (int64_t)k * BLK, e = b + BLK;
         std::sort(b, std::find(b, e, -1));
     }
+    return true;
+}
+
+static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
+    const int64_t m = X.nr;
+    if (m != X.nc || m < 2 || m >= (1ll << 31)) return false;
+    std::vector<int32_t> lanes;
+    int spb = 4;
+    if (g_sell_cline && dt == QOB_C64 && closed_lanes(X, 8, 320, lanes) && (double)m >= 0.9 * (double)lanes.size())
+        spb = 5;
+    else if (!closed_lanes(X, 1, 256, lanes))
+        return false;
+    const int BLK = 64 * spb;
+    const int nblocks = (int)(lanes.size() / BLK), nslices = nblocks * spb;
     std::vector<int32_t> lane_of((size_t)m, -1), off(nslices + 1, 0);
     for (size_t e = 0; e < lanes.size(); ++e)
         if (lanes[e] >= 0) lane_of[lanes[e]] = (int32_t)(e % BLK);
@@ -321,6 +347,7 @@ static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
     S.dev.col = (const int32_t *)S.col.p;
     S.dev.val = S.val.p;
     S.dev.rows = (const int32_t *)S.rows.p;
+    S.dev.spb = spb;
     return true;
 }
 
@@ -2770,7 +2797,7 @@ static int *tune_slot(const char *name) {
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
-               {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed},
+               {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed}, {"sell_cline", &g_sell_cline},
                {"sell_cjchunk", &g_sell_cjchunk}, {"csc_groups", &g_csc_groups},
                {"csc_gkernel", &g_csc_gkernel},   {"csc_gremap", &g_csc_gremap},
                {"sell_cremap", &g_sell_cremap},   {"ls_materialize", &g_ls_materialize},
@@ -2789,8 +2816,9 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "sell_cj") && value != 1 && value != 2 && value != 4 && value != 8)
         return fail(QOB_EARG, "sell_cj must be 1, 2, 4 or 8");
     if (!strcmp(name, "qubit_tbits") && (value < 10 || value > 13)) return fail(QOB_EARG, "qubit_tbits must be in 10..13");
-    if (!strcmp(name, "mode_kernel") && value != 0 && value != 1)
-        return fail(QOB_EARG, "mode_kernel must be 0 (automatic) or 1 (tiled GEMM for every dense mode product)");
+    if (!strcmp(name, "mode_kernel") && value > 2)
+        return fail(QOB_EARG, "mode_kernel must be 0 (automatic), 1 (tiled GEMM for every dense mode product) or 2 "
+                              "(fibre-group kernel for leading dimensions below 16 as well)");
     if (!strcmp(name, "sell_unr") && value != 2 && value != 4 && value != 8)
         return fail(QOB_EARG, "sell_unr must be 2, 4 or 8");
     if (!strcmp(name, "csc_sweep") && value > 2) return fail(QOB_EARG, "csc_sweep must be 0, 1 or 2");
@@ -2803,6 +2831,7 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
     if (!strcmp(name, "qubit_remap") && value > 2) return fail(QOB_EARG, "qubit_remap must be 0, 1 or 2");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
+         !strcmp(name, "sell_cline") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
          !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles") ||
          !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg") || !strcmp(name, "qubit_ntfar")) &&

@@ -57,12 +57,20 @@ __device__ void stage_factor(T *Fs, int ldf, int rows, int cols, const T *__rest
 
 // ------------------------------------------------------------ k_mode_fib
 // NT16 = ceil(max(d_in, d_out) / 16): output tiles per fibre group and
-// k-steps / 4.
+// k-steps / 4.  A fibre group is 16 fibres: with n1 >= 16, 16 consecutive
+// i1 of one i3 (gpi groups per i3); with n1 < 16 (flat), 16 consecutive
+// fibres f = i1 + n1 * i3 of the nf = n1 * n3 fibres.  The loads of the next
+// chunk of 16 k -- the next group's first chunk after a group's last -- are
+// in flight during the current chunk's MFMAs and the group's stores.
+struct FibreAt {
+    const void *p;  // first element of the fibre (in), or null past the end
+    int64_t o;      // offset of the fibre in out
+};
 template <typename R, int NT16>
 __global__ void __launch_bounds__(MODE_NT) k_mode_fib(const cplx<R> *__restrict__ F, int d_out, int d_in,
                                                        const cplx<R> *__restrict__ in, cplx<R> *__restrict__ out,
-                                                       int n1, int gpi, int64_t ngroups, double2 alpha_,
-                                                       double2 beta_, int beta_zero) {
+                                                       int n1, int gpi, int64_t ngroups, int64_t nf,
+                                                       double2 alpha_, double2 beta_, int beta_zero, int flat) {
     using T = cplx<R>;
     using V = typename mf<R>::v4;
     constexpr int KS = 4 * NT16, LDF = 16 * NT16 + 1;
@@ -75,34 +83,49 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_fib(const cplx<R> *__restrict_
     const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
     const int64_t sin = (int64_t)n1 * d_in, sout = (int64_t)n1 * d_out;
     constexpr int WPB = MODE_NT / 64;
-    for (int64_t g = (int64_t)blockIdx.x * WPB + wave; g < ngroups; g += (int64_t)gridDim.x * WPB) {
-        const int64_t i3 = g / gpi;
-        const int i1 = (int)(g - i3 * gpi) * 16 + lr;
-        const bool ok = i1 < n1;
-        const T *ib = in + (ok ? i1 + i3 * sin : 0);
+    const int64_t gstep = (int64_t)gridDim.x * WPB;
+    // this lane's fibre of group g
+    auto fibre = [&](int64_t g) -> FibreAt {
+        int64_t i3;
+        int i1;
+        bool ok;
+        if (g >= ngroups) return {nullptr, 0};
+        if (flat) {
+            const int64_t f = g * 16 + lr;
+            ok = f < nf;
+            i3 = n1 == 1 ? f : f / n1;
+            i1 = (int)(f - i3 * n1);
+        } else {
+            i3 = g / gpi;
+            i1 = (int)(g - i3 * gpi) * 16 + lr;
+            ok = i1 < n1;
+        }
+        if (!ok) return {nullptr, 0};
+        return {in + i1 + i3 * sin, i1 + i3 * sout};
+    };
+    auto load = [&](const FibreAt &fa, int kc, T (&x)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = 16 * kc + 4 * j + lk;
+            x[j] = (fa.p && b < d_in) ? ((const T *)fa.p)[(int64_t)n1 * b] : czero<T>();
+        }
+    };
+    int64_t g = (int64_t)blockIdx.x * WPB + wave;
+    FibreAt cur = fibre(g);
+    T x[4], xn[4];
+    load(cur, 0, x);
+    for (; g < ngroups; g += gstep) {
+        const FibreAt nxt = fibre(g + gstep);
         V ar[NT16], ai[NT16];
 #pragma unroll
         for (int t = 0; t < NT16; ++t) {
             ar[t] = V{0, 0, 0, 0};
             ai[t] = V{0, 0, 0, 0};
         }
-        // k in chunks of 16 (four k-steps); the next chunk's loads are in
-        // flight during this chunk's MFMAs
-        T x[4], xn[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int b = 4 * j + lk;
-            x[j] = (ok && b < d_in) ? ib[(int64_t)n1 * b] : czero<T>();
-        }
 #pragma unroll 1
         for (int kc = 0; kc < NT16; ++kc) {
-            if (kc + 1 < NT16) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int b = 16 * (kc + 1) + 4 * j + lk;
-                    xn[j] = (ok && b < d_in) ? ib[(int64_t)n1 * b] : czero<T>();
-                }
-            }
+            if (kc + 1 < NT16) load(cur, kc + 1, xn);
+            else load(nxt, 0, xn);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
 #pragma unroll
@@ -112,8 +135,8 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_fib(const cplx<R> *__restrict_
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[j] = xn[j];
         }
-        if (ok) {
-            T *ob = out + i1 + i3 * sout;
+        if (cur.p) {
+            T *ob = out + cur.o;
 #pragma unroll
             for (int t = 0; t < NT16; ++t) {
 #pragma unroll
@@ -128,6 +151,7 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_fib(const cplx<R> *__restrict_
                 }
             }
         }
+        cur = nxt;
     }
 }
 
@@ -465,7 +489,10 @@ template <typename K> hipError_t raise_lds(K kfn, size_t bytes) {
 
 }  // namespace
 
-int g_mode_kernel = 0;  // 0: automatic; 1: always the tiled GEMM (for tests and timing)
+// 0: automatic (fibre groups for n1 >= 16, LDS chunks below); 1: always the
+// tiled GEMM; 2: flat fibre groups for n1 < 16 as well (1 and 2 for tests
+// and timing; the LDS chunks measured 2% faster on a (16, 32, 16) chain)
+int g_mode_kernel = 0;
 
 int64_t zgemv_splits(int64_t m, int64_t n) {
     const int64_t rt = (m + 63) / 64;
@@ -566,8 +593,8 @@ hipError_t launch_mode(qob_dtype dt, const void *F, int64_t d_out, int64_t d_in,
                        int64_t n1, int64_t n3, double2 alpha, double2 beta, hipStream_t s, void *ws) {
     if (n1 <= 0 || n3 <= 0 || d_out <= 0) return hipSuccess;
     if (d_in <= 0) return hipErrorInvalidValue;
-    const bool small = d_in <= 64 && d_out <= 64 && g_mode_kernel == 0;
-    if (n1 == 1 && n3 == 1 && g_mode_kernel == 0) return launch_zgemv(dt, F, d_out, d_in, in, out, ws, alpha, beta, s);
+    const bool small = d_in <= 64 && d_out <= 64 && g_mode_kernel != 1;
+    if (n1 == 1 && n3 == 1 && g_mode_kernel != 1) return launch_zgemv(dt, F, d_out, d_in, in, out, ws, alpha, beta, s);
     if (!small) {
         if (n1 == 1)  // out (d_out x n3) = F in
             return launch_zgemm(dt, d_out, n3, d_in, F, 1, d_out, 0, in, 1, d_in, 0, out, 1, d_out, 0, 1, alpha,
@@ -584,8 +611,11 @@ hipError_t launch_mode(qob_dtype dt, const void *F, int64_t d_out, int64_t d_in,
     const size_t esz = dt == QOB_C64 ? 8 : 4;
     const size_t fbytes = (size_t)(4 * 4 * nt16) * (16 * nt16 + 1) * 2 * esz;
     const int ncu = device_cu_count();
-    if (n1 >= 16) {
-        const int64_t gpi = (n1 + 15) / 16, ng = gpi * n3;
+    if (n1 >= 16 || g_mode_kernel == 2) {
+        // n1 >= 16: groups of 16 consecutive i1; else of 16 consecutive fibres
+        const int flat = n1 < 16;
+        const int64_t nf = n1 * n3;
+        const int64_t gpi = (n1 + 15) / 16, ng = flat ? (nf + 15) / 16 : gpi * n3;
 #define QOB_MF(RR, NT)                                                                                          \
     do {                                                                                                        \
         auto k = k_mode_fib<RR, NT>;                                                                            \
@@ -594,7 +624,8 @@ hipError_t launch_mode(qob_dtype dt, const void *F, int64_t d_out, int64_t d_in,
         const int64_t grid = std::max<int64_t>(                                                                 \
             1, std::min<int64_t>((ng + 7) / 8, (int64_t)ncu * resident_per_cu(k, MODE_NT, fbytes)));           \
         hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(MODE_NT), fbytes, s, (const cplx<RR> *)F, (int)d_out, \
-                           (int)d_in, (const cplx<RR> *)in, (cplx<RR> *)out, (int)n1, (int)gpi, ng, alpha, beta, bz); \
+                           (int)d_in, (const cplx<RR> *)in, (cplx<RR> *)out, (int)n1, (int)gpi, ng, nf, alpha, beta, \
+                           bz, flat);                                                                          \
         return hipGetLastError();                                                                               \
     } while (0)
         if (n1 > 0x7fffffff) return hipErrorInvalidValue;

@@ -122,6 +122,47 @@ __device__ __forceinline__ void qb_uniform(T wt, const char *basep, uint32_t tpb
     }
 }
 
+// acc += w * x[k ^ f] for every element k of the lane: a flip of tile bits
+// at or above log2(NTH) (F = f NTH) pairs elements of one lane, read from
+// the lane's registers; f = 0 is the diagonal.  f is wave-uniform, so the
+// branch per value keeps every register index a compile-time constant.
+template <typename T, int EPT>
+__device__ __forceinline__ void qb_regflip(T wt, const T (&xv)[EPT], uint32_t f, T (&acc)[EPT]) {
+#pragma unroll
+    for (int ff = 0; ff < EPT; ++ff) {
+        if (f == (uint32_t)ff) {
+            if (wt.y == 0) {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    acc[k].x = fma(wt.x, xv[k ^ ff].x, acc[k].x);
+                    acc[k].y = fma(wt.x, xv[k ^ ff].y, acc[k].y);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) acc[k] = cfma(wt, xv[k ^ ff], acc[k]);
+            }
+        }
+    }
+}
+
+// the same with a per-element weight; a zero weight selects (a NaN partner
+// must not propagate)
+template <typename T, int EPT>
+__device__ __forceinline__ void qb_regflip_w(const T (&wt)[EPT], const T (&xv)[EPT], uint32_t f, T (&acc)[EPT]) {
+#pragma unroll
+    for (int ff = 0; ff < EPT; ++ff) {
+        if (f == (uint32_t)ff) {
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                const T v = cfma(wt[k], xv[k ^ ff], acc[k]);
+                const bool nz = wt[k].x != 0 || wt[k].y != 0;
+                acc[k].x = nz ? v.x : acc[k].x;
+                acc[k].y = nz ? v.y : acc[k].y;
+            }
+        }
+    }
+}
+
 // Tile coordinates of the first pass: tile element e is basis state base + e.
 struct LoMap {
     uint32_t base, lowmask;
@@ -152,12 +193,15 @@ template <int TBITS> struct HiMap {
 // partial tile never leave it (the state is a single tile then).
 template <typename T, int NTH, int EPT, bool LO, class Map>
 __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *__restrict__ vals, const T *tile,
-                                       const T *__restrict__ in, const Map &m, T (&acc)[EPT],
+                                       const T *__restrict__ in, const Map &m, const T (&xv)[EPT], T (&acc)[EPT],
                                        uint32_t far_tiles = 0xffffffffu) {
     const uint32_t tid = threadIdx.x;
     const int nconst = pg[0];
     const int nwg = pg[1];
     const int32_t *r = pg + 4;
+    // register flips in ComplexF32 only: the lane's copy of its elements
+    // costs ComplexF64 16 VGPRs and 5% (measured), ComplexF32 gains 1.5%
+    constexpr bool REG = sizeof(T) == 8;
     if constexpr (LO) {
         // Partner values of group gr into dst, from global memory: the tile
         // of basis states (base + e) ^ F = (base ^ Fhigh) + (e ^ Flow); also
@@ -204,8 +248,11 @@ __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *
                 const int32_t *rn = r + 8 * (g + 1 < nin ? g + 1 : g);
                 const uint32_t Fn = (uint32_t)__builtin_amdgcn_readfirstlane(rn[0]);
                 const T wn = rec_weight_s<T>(rn + 4);
-                qb_uniform<T, NTH, EPT>(wt, (const char *)tile,
-                                        ((tid ^ m.fq(F)) & m.lowmask) * (uint32_t)sizeof(T), acc);
+                if (REG && (F & (uint32_t)(NTH - 1)) == 0)  // flips of register bits only
+                    qb_regflip<T, EPT>(wt, xv, (F & m.lowmask) / (uint32_t)NTH, acc);
+                else
+                    qb_uniform<T, NTH, EPT>(wt, (const char *)tile,
+                                            ((tid ^ m.fq(F)) & m.lowmask) * (uint32_t)sizeof(T), acc);
                 F = Fn;
                 wt = wn;
             }
@@ -280,7 +327,9 @@ __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *
         }
         // a zero weight must not propagate a NaN / Inf partner (rows absent
         // from the factors): select instead of multiply
-        if (m.in_tile(F)) {
+        if (REG && LO && m.in_tile(F) && (F & (uint32_t)(NTH - 1)) == 0) {
+            qb_regflip_w<T, EPT>(wt, xv, (F & m.lowmask) / (uint32_t)NTH, acc);
+        } else if (m.in_tile(F)) {
             const uint32_t tpb = (tid ^ m.fq(F)) * (uint32_t)sizeof(T);
 #pragma unroll
             for (int k = 0; k < EPT; ++k) {
@@ -350,25 +399,31 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
         const bool add_out = S.prog_hi_off >= 0;
         const bool bz = beta_zero && !add_out;
         const T beff = add_out ? mk((typename real_of<T>::type)1, (typename real_of<T>::type)0) : beta;
-        __syncthreads();  // the previous item's tile reads are finished
+        // the lane keeps its own elements in registers as well: flips of
+        // the register bits (e = tid + k NTH, k = 0..EPT-1), the diagonal
+        // and the gamma term read them there
+        T xv[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
-            if (e < tsize) tile[e] = ldg(in + base + e);
+            xv[k] = e < tsize ? ldg(in + base + e) : czero<T>();
         }
+        __syncthreads();  // the previous item's tile reads are finished
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) tile[tid + k * NTH] = xv[k];
         __syncthreads();
         T acc[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
         const LoMap m{base, (uint32_t)tsize - 1u, tb};
-        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, in, m, acc, far_tiles);
+        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, in, m, xv, acc, far_tiles);
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
             if (e < tsize) {
                 const uint32_t J = base + (uint32_t)e;
                 T r = cmul(alpha, acc[k]);
-                if (!gamma_zero) r = cfma(gamma, tile[e], r);
+                if (!gamma_zero) r = cfma(gamma, sizeof(T) == 8 ? xv[k] : tile[e], r);
                 if (!bz) r = cfma(beff, ldg(out + J), r);
                 stg_stream(out + J, r);
             }
@@ -424,7 +479,8 @@ __global__ void __launch_bounds__(NTH) k_qubit_hi(const QubitStateDev *__restric
         T acc[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
-        qb_run<T, NTH, EPT, false>(prog + S.prog_hi_off, vals, tile, in, m, acc);
+        const T(&xv)[EPT] = acc;  // not read: the second pass has no register flips
+        qb_run<T, NTH, EPT, false>(prog + S.prog_hi_off, vals, tile, in, m, xv, acc);
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const uint32_t J = m.J((uint32_t)(tid + k * NTH));
@@ -454,18 +510,21 @@ __global__ void __launch_bounds__(NTH) k_qubit_dot(QubitStateDev S, const T *__r
     double rx = 0, ry = 0;
     for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
         const uint32_t base = (uint32_t)it << tb;
-        __syncthreads();
+        T xv[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;
-            if (e < tsize) tile[e] = ldg(v + base + e);
+            xv[k] = e < tsize ? ldg(v + base + e) : czero<T>();
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) tile[tid + k * NTH] = xv[k];
         __syncthreads();
         T acc[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
         const LoMap m{base, (uint32_t)tsize - 1u, tb};
-        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, v, m, acc);
+        qb_run<T, NTH, EPT, true>(prog + S.prog_off, vals, tile, v, m, xv, acc);
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NTH;

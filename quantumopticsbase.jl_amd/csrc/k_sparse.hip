@@ -142,29 +142,31 @@ __global__ void __launch_bounds__(256) k_sell_left_rr(int nslices, int64_t m, co
     }
 }
 
-// ---- left, closed blocks: lane t of workgroup k owns row rows[256k + t]
-// of R and of B (-1: idle).  The block's rows are whole connected components
-// of the pattern of X, so each entry of X in the block names a lane of the
-// same block (col = that lane).  Per batch of UNR columns every lane loads
-// its own row of B once into the workgroup's LDS and the entries read from
-// there: each element of B is loaded once, with no re-reads through L2.
-template <typename T, int MAXW, int UNR>
-__global__ void __launch_bounds__(256) k_sell_left_closed(int nslices, const int32_t *__restrict__ slice_off,
-                                                          const int32_t *__restrict__ rows,
-                                                          const int32_t *__restrict__ col, const T *__restrict__ val,
-                                                          const T *__restrict__ B, int64_t ldb, int64_t cols,
-                                                          T *__restrict__ R, int64_t ldr, double2 alpha_,
-                                                          double2 beta_, int beta_zero, uint32_t nsg, int jchunk,
-                                                          int remap) {
-    __shared__ T sb[UNR][256];
+// ---- left, closed blocks: lane t of workgroup k owns row rows[BLK k + t]
+// of R and of B (-1: idle), BLK = 64 SPB.  The block's rows are whole
+// connected components of the pattern of X, so each entry of X in the block
+// names a lane of the same block (col = that lane).  Per batch of UNR
+// columns every lane loads its own row of B once into the workgroup's LDS
+// and the entries read from there: each element of B is loaded once, with no
+// re-reads through L2.
+template <typename T, int MAXW, int UNR, int SPB>
+__global__ void __launch_bounds__(64 * SPB) k_sell_left_closed(int nslices, const int32_t *__restrict__ slice_off,
+                                                               const int32_t *__restrict__ rows,
+                                                               const int32_t *__restrict__ col,
+                                                               const T *__restrict__ val, const T *__restrict__ B,
+                                                               int64_t ldb, int64_t cols, T *__restrict__ R,
+                                                               int64_t ldr, double2 alpha_, double2 beta_,
+                                                               int beta_zero, uint32_t nsg, int jchunk, int remap) {
+    constexpr int BLK = 64 * SPB;
+    __shared__ T sb[UNR][BLK];
     const uint32_t G = gridDim.x;
     const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
     const uint32_t sg = lb % nsg, cg = lb / nsg;
     const int t = threadIdx.x;
-    if ((int64_t)sg * 4 >= nslices) return;  // workgroup-uniform, before any barrier
-    const int slice = (int)(sg * 4 + (t >> 6));  // nslices is a multiple of 4: every lane has a slice
+    if ((int64_t)sg * SPB >= nslices) return;  // workgroup-uniform, before any barrier
+    const int slice = (int)(sg * SPB + (t >> 6));  // nslices is a multiple of SPB: every lane has a slice
     const int r = t & 63;
-    const int32_t row = rows[(int64_t)sg * 256 + t];
+    const int32_t row = rows[(int64_t)sg * BLK + t];
     const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_);
     const int32_t o0 = slice_off[slice];
     const int w = (slice_off[slice + 1] - o0) >> 6;
@@ -476,10 +478,17 @@ static void sell_rr(const SellDev &X, const void *B, int64_t cols, void *R, doub
 template <typename T>
 static void sell_closed(const SellDev &X, const void *B, int64_t cols, void *R, double2 alpha, double2 beta, int bz,
                         hipStream_t s, uint32_t nsg, uint32_t G, int jchunk) {
-#define QOB_CL(MW, U)                                                                                           \
-    hipLaunchKernelGGL((k_sell_left_closed<T, MW, U>), dim3(G), dim3(256), 0, s, X.nslices, X.slice_off, X.rows, \
-                       X.col, (const T *)X.val, (const T *)B, X.n, cols, (T *)R, X.m, alpha, beta, bz, nsg, jchunk, \
-                       g_sell_cremap)
+#define QOB_CL(MW, U)                                                                                          \
+    do {                                                                                                       \
+        if (X.spb == 5)                                                                                        \
+            hipLaunchKernelGGL((k_sell_left_closed<T, MW, U, 5>), dim3(G), dim3(320), 0, s, X.nslices,         \
+                               X.slice_off, X.rows, X.col, (const T *)X.val, (const T *)B, X.n, cols, (T *)R,   \
+                               X.m, alpha, beta, bz, nsg, jchunk, g_sell_cremap);                              \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_sell_left_closed<T, MW, U, 4>), dim3(G), dim3(256), 0, s, X.nslices,         \
+                               X.slice_off, X.rows, X.col, (const T *)X.val, (const T *)B, X.n, cols, (T *)R,   \
+                               X.m, alpha, beta, bz, nsg, jchunk, g_sell_cremap);                              \
+    } while (0)
 #define QOB_CL_U(MW)                         \
     do {                                     \
         if (jchunk == 1) QOB_CL(MW, 1);      \
@@ -499,8 +508,8 @@ hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64
     if (X.m == 0 || cols == 0) return hipSuccess;
     const int bz = (beta.x == 0.0 && beta.y == 0.0) | (g_nt_store << 1);
     if (X.rows) {  // closed blocks (built only for square X, at most 16 entries per row)
-        if (X.maxw > 16 || X.nslices % 4) return hipErrorInvalidValue;
-        const uint32_t nsg = (uint32_t)(X.nslices / 4);
+        if (X.maxw > 16 || (X.spb != 4 && X.spb != 5) || X.nslices % X.spb) return hipErrorInvalidValue;
+        const uint32_t nsg = (uint32_t)(X.nslices / X.spb);
         const int jchunk = (int)std::max<int64_t>(1, std::min<int64_t>(g_sell_cjchunk, cols));
         const uint64_t ncg = (uint64_t)((cols + jchunk - 1) / jchunk);
         const uint64_t G = (uint64_t)nsg * ncg;

@@ -207,10 +207,12 @@ struct SellDev {
     const int32_t *col = nullptr;
     const void *val = nullptr;            // CF64 or CF32
     // Closed blocks (rows != nullptr): slice k holds rows rows[64k + lane]
-    // (-1: idle lane); blocks of 4 slices hold whole connected components of
-    // the pattern of X, and col[] holds the lane (0..255) of the block that
-    // owns the needed row of B.  nslices is a multiple of 4.
+    // (-1: idle lane); blocks of spb slices (4 or 5) hold whole connected
+    // components of the pattern of X, and col[] holds the lane (0..64 spb - 1)
+    // of the block that owns the needed row of B.  nslices is a multiple of
+    // spb.
     const int32_t *rows = nullptr;
+    int spb = 4;
 };
 // CSC of X (columns of X as lists of (row, value)).
 struct CscDev {

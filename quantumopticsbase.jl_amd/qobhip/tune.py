@@ -6,7 +6,8 @@ of libqob_hip.so (qob_tune_set / qob_tune_get):
   mode products     fused_gpw (fibre groups in flight per wave in the
                     fused MFMA kernel; 0 = by precision and factor width),
                     fused_resident (factors kept in LDS for the launch);
-                    mode_kernel (1 = tiled GEMM for every dense mode product,
+                    mode_kernel (1 = tiled GEMM for every dense mode product, 2 = fibre-group
+                    kernel for leading dimensions below 16 as well,
                     a test and timing switch, not tuned)
   qubit sums        qubit_tbits (tile = 2^tbits states), qubit_nth,
                     qubit_grid (persistent workgroups; 0 = 2 per CU for ComplexF64, 4 for ComplexF32),
@@ -20,8 +21,10 @@ of libqob_hip.so (qob_tune_set / qob_tune_get):
                     (component-group right kernel, its XCD remap),
                     sell_cjchunk / sell_cremap (closed-block left kernel),
                     nt_store; sell_align (diagonal-aligned slots),
-                    sell_closed (closed blocks) and csc_groups (component
-                    groups) apply to tables built after they are set
+                    sell_closed (closed blocks), sell_cline (closed blocks
+                    of whole 128-byte lines, 320 rows) and csc_groups
+                    (component groups) apply to tables built after they are
+                    set
 
 ``tune`` runs a coordinate search: starting from the current values it
 tries every value of one tunable with the others fixed, keeps the fastest,

@@ -167,8 +167,10 @@ def test_lazytensor_dense_factors_any_width(case, dtype):
 @pytest.mark.parametrize("case", [0, 1, 3, 4, 10])
 @pytest.mark.parametrize("dtype", DTS)
 def test_small_mode_kernels_equal_gemm_path(case, dtype):
-    """The fibre and chunk kernels against the tiled GEMM on the same mode
-    products (mode_kernel = 1 sends every dense mode product to the GEMM)."""
+    """The fibre and LDS-chunk kernels (mode_kernel = 0) and the fibre
+    kernel also for leading dimensions below 16 (mode_kernel = 2) against
+    the tiled GEMM on the same mode products (mode_kernel = 1 sends every
+    dense mode product to the GEMM)."""
     shl, shr, idx, kinds = WIDE_CASES[case]
     g = np.random.default_rng(400 + case)
     oop = rand_lt(g, shl, shr, idx, kinds)
@@ -178,7 +180,7 @@ def test_small_mode_kernels_equal_gemm_path(case, dtype):
     outs = []
     fused = _tune("fused_lt", 0)
     try:
-        for mk in (0, 1):
+        for mk in (0, 1, 2):
             old = _tune("mode_kernel", mk)
             try:
                 R = Q.Operator(qop.basis_l, gb(21), Q.DeviceArray(int(np.prod(shl)) * 21, dtype, (int(np.prod(shl)), 21)))
@@ -189,6 +191,7 @@ def test_small_mode_kernels_equal_gemm_path(case, dtype):
     finally:
         _tune("fused_lt", fused)
     assert rel(outs[0], outs[1]) < TOLS[dtype]
+    assert rel(outs[2], outs[1]) < TOLS[dtype]
 
 
 @pytest.mark.parametrize("transa", [0, 1])

@@ -711,6 +711,87 @@ def test_sparse_left_closed_slices(sizes, adj, dtype, tol):
         assert np.array_equal(outs[0], o)
 
 
+def _line_groups_matrix(g, n=1280, shift=643):
+    """Square sparse X whose rows, joined within aligned groups of 8 as
+    well as by the pattern, form components that pack 320-row blocks at
+    least 90% full: a path inside every aligned group of 8 rows and a
+    coupling of row i with row i + shift (an offset that is not a multiple
+    of 8) for the first row of every group of the first half, so each
+    component is two groups of 8."""
+    rows, cols = [], []
+    for i in range(n):
+        rows.append(i); cols.append(i)
+        if i % 8 != 7:
+            rows.append(i); cols.append(i + 1)
+    for i in range(0, n // 2, 8):
+        rows.append(i); cols.append(i + shift)
+    v = g.standard_normal(len(rows)) + 1j * g.standard_normal(len(rows))
+    X = sp.csc_matrix((v, (rows, cols)), shape=(n, n))
+    X.sort_indices()
+    return X
+
+
+def _line_layout_fill(X, grp=8, blk=320):
+    """Largest component and block fill of the line-aligned closed layout."""
+    from scipy.sparse.csgraph import connected_components
+    n = X.shape[0]
+    A = (abs(X) + abs(X.T)).tocsr()
+    i = np.arange(n)
+    G = sp.csr_matrix((np.ones(n), (i, i - i % grp)), shape=(n, n))
+    _, lab = connected_components((A + G + G.T).tocsr(), directed=False)
+    sizes = np.bincount(lab)
+    first = {}
+    for r in range(n):
+        first.setdefault(lab[r], r)
+    blocks, used = 0, blk
+    for c in sorted(first, key=first.get):
+        if used + sizes[c] > blk:
+            blocks, used = blocks + 1, 0
+        used += sizes[c]
+    return sizes.max(), n / (blocks * blk)
+
+
+@pytest.mark.parametrize("adj", [False, True])
+def test_sparse_left_closed_line_aligned(adj):
+    """ComplexF64 closed blocks whose rows are also joined within aligned
+    8-row groups (every run of a block is whole 128-byte lines of R), packed
+    into 320-row blocks: against the oracle, and bit for bit against the
+    256-row closed form and the contiguous slices."""
+    import ctypes as C
+    from qobhip import _lib
+    g = np.random.default_rng(23)
+    X = _line_groups_matrix(g)
+    big, fill = _line_layout_fill(X)
+    assert big <= 320 and fill >= 0.9, (big, fill)  # the line-aligned form applies
+    n, cols = X.shape[0], 9
+    oop = O.OSparse.from_scipy(X)
+    if adj:
+        oop = O.dagger(oop)
+    B = g.standard_normal((n, cols)) + 1j * g.standard_normal((n, cols))
+    want = np.zeros((n, cols), complex)
+    O.mul_op_left(want, oop, B, 1.0, 0.0)
+    saved = {}
+    for name in (b"sell_closed", b"sell_cline"):
+        v = C.c_int64()
+        _lib.call("qob_tune_get", name, C.byref(v))
+        saved[name] = v.value
+    outs = []
+    try:
+        for closed, cline in ((1, 1), (1, 0), (0, 0)):
+            _lib.call("qob_tune_set", b"sell_closed", closed)
+            _lib.call("qob_tune_set", b"sell_cline", cline)
+            qop = to_q(oop)   # tables are built on first use, under these settings
+            R = Q.Operator(gb(n), gb(cols), np.full((n, cols), np.nan, complex))
+            Q.mul_(R, qop, Q.Operator(gb(n), gb(cols), B), 1.0, 0.0)
+            assert rel(R.to_numpy(), want) < TOL64
+            outs.append(R.to_numpy())
+    finally:
+        for name, v in saved.items():
+            _lib.call("qob_tune_set", name, v)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
 @pytest.mark.parametrize("sizes", [[1, 2, 3, 4] * 50, [8, 7, 1, 5] * 20, [2] * 60, [9] + [3] * 40])
 @pytest.mark.parametrize("adj", [False, True])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
