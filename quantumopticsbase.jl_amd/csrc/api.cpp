@@ -2837,8 +2837,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
          !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg") || !strcmp(name, "qubit_ntfar")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
-    if (!strcmp(name, "qubit_nth") && value != 256 && value != 512 && value != 1024)
-        return fail(QOB_EARG, "qubit_nth must be 256, 512 or 1024");
+    if (!strcmp(name, "qubit_nth") && value != 0 && value != 256 && value != 512 && value != 1024)
+        return fail(QOB_EARG, "qubit_nth must be 0 (by element type), 256, 512 or 1024");
     if ((!strcmp(name, "csc_cc") || !strcmp(name, "sell_jchunk") || !strcmp(name, "sell_cjchunk")) && value < 1)
         return fail(QOB_EARG, "%s must be >= 1", name);
     *p = (int)value;

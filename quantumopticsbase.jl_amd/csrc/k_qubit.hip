@@ -573,7 +573,11 @@ __global__ void __launch_bounds__(NTH) k_qubit_axpy(const QubitStateDev *__restr
 
 }  // namespace
 
-int g_qubit_nth = 512;
+// threads per workgroup; 0: 512 for ComplexF64, 256 for ComplexF32 (measured
+// fastest on the 256-chain batch: 4 elements per lane in ComplexF64, 8 in
+// ComplexF32 at 2^11-element tiles)
+int g_qubit_nth = 0;
+static int qubit_nth(qob_dtype dt) { return g_qubit_nth ? g_qubit_nth : (dt == QOB_C64 ? 512 : 256); }
 // 0: workgroup b walks items b, b + G, ...; 1: the same rounds of G items with
 // each XCD given a contiguous run of every round; 2: each XCD walks its own
 // item range for the whole launch (a batch's states are assigned to XCDs
@@ -664,8 +668,8 @@ static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, co
     } while (0)
 #define QOB_QB_NT(TT, TB)                                  \
     do {                                                   \
-        if (g_qubit_nth == 512) QOB_QB(TT, 512, TB);       \
-        else if (g_qubit_nth == 1024) QOB_QB(TT, 1024, TB); \
+        if (qubit_nth(dt) == 512) QOB_QB(TT, 512, TB);       \
+        else if (qubit_nth(dt) == 1024) QOB_QB(TT, 1024, TB); \
         else QOB_QB(TT, 256, TB);                          \
     } while (0)
 #define QOB_QB_TB(TT)                          \
@@ -709,8 +713,8 @@ static hipError_t launch_qubit_hi_any(qob_dtype dt, const QubitStateDev *states,
     } while (0)
 #define QOB_QH_NT(TT, TB)                                  \
     do {                                                   \
-        if (g_qubit_nth == 512) QOB_QH(TT, 512, TB);       \
-        else if (g_qubit_nth == 1024) QOB_QH(TT, 1024, TB); \
+        if (qubit_nth(dt) == 512) QOB_QH(TT, 512, TB);       \
+        else if (qubit_nth(dt) == 1024) QOB_QH(TT, 1024, TB); \
         else QOB_QH(TT, 256, TB);                          \
     } while (0)
 #define QOB_QH_TB(TT)                            \
