@@ -190,13 +190,35 @@ static RowLists identity_rows(int64_t m, int64_t n) {
 enum { V_N = 0, V_T = 1, V_CJ = 2, V_H = 3 };
 
 // --------------------------------------------------------------- sparse op
+struct RefreshMap;
+// src: for every value slot of the table, the entry of the row lists it was
+// built from (-1: a padding zero); rmap: the slot values as linear functions
+// of a LazySum's coefficients (materialised sums under device-held
+// coefficients, built on first need)
 struct SellTab {
     DevBuf off, col, val, rows;
     SellDev dev;
+    std::vector<int64_t> src;
+    std::shared_ptr<RefreshMap> rmap;
 };
 struct CscTab {
     DevBuf ptr, row, val, ord, gint, gcoef;
     CscDev dev;
+    std::vector<int64_t> gsrc;  // of gcoef (val is the row lists' values in order)
+    std::shared_ptr<RefreshMap> rmap, grmap;
+};
+
+// A materialised LazySum's entries as linear functions of its coefficients:
+// entry e of the summed CSR matrix is sum over c in [cptr[e], cptr[e + 1]) of
+// coef[cterm[c]] * cbase[c]; csc2csr maps the CSC entries (transpose) to
+// CSR entries.  dcoef / dcoef_on are the owner's device coefficients.
+struct MatContrib {
+    std::vector<int64_t> cptr;
+    std::vector<int32_t> cterm;
+    std::vector<std::complex<double>> cbase;
+    std::vector<int64_t> csc2csr;
+    const void *const *dcoef = nullptr;
+    const bool *dcoef_on = nullptr;
 };
 
 struct SparseData {
@@ -206,6 +228,7 @@ struct SparseData {
     std::mutex mu;
     std::map<int, std::shared_ptr<SellTab>> sell;   // key: var*2 + dt
     std::map<int, std::shared_ptr<CscTab>> cols;    // key: var*2 + dt
+    std::shared_ptr<MatContrib> mc;                 // materialised LazySum only
 
     // rows of op_v(M)
     RowLists rows(int v) const {
@@ -324,6 +347,7 @@ static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
     }
     std::vector<int32_t> col(off[nslices], -1);
     std::vector<cd> val(off[nslices], cd(0, 0));
+    S.src.assign(off[nslices], -1);
     for (int k = 0; k < nslices; ++k)
         for (int l = 0; l < 64; ++l) {
             const int32_t r = lanes[(size_t)k * 64 + l];
@@ -333,6 +357,7 @@ static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
                 const int64_t e = off[k] + (int64_t)p * 64 + l;
                 col[e] = lane_of[X.idx[q]];
                 val[e] = X.val[q];
+                S.src[e] = q;
             }
         }
     if (S.off.upload(off.data(), off.size() * 4) != QOB_OK || S.col.upload(col.data(), col.size() * 4) != QOB_OK ||
@@ -376,6 +401,7 @@ static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
     }
     std::vector<int32_t> col(off[nslices], -1);
     std::vector<cd> val(off[nslices], cd(0, 0));
+    S.src.assign(off[nslices], -1);
     for (int k = 0; k < nslices; ++k) {
         const std::vector<int64_t> &d = diag[k];
         for (int64_t r = (int64_t)k * 64; r < std::min<int64_t>(m, (int64_t)k * 64 + 64); ++r) {
@@ -387,6 +413,7 @@ static qob_status build_sell(const RowLists &X, qob_dtype dt, SellTab &S) {
                 const int64_t e = off[k] + (int64_t)slot * 64 + (r - (int64_t)k * 64);
                 col[e] = X.idx[q];
                 val[e] = X.val[q];
+                S.src[e] = q;
             }
         }
     }
@@ -564,6 +591,7 @@ static bool build_csc_groups(const RowLists &Xcols, int64_t xrows, qob_dtype dt,
     for (int64_t i = 0; i < n; ++i) members[cur[find((int32_t)i)]++] = (int32_t)i;
     std::vector<int32_t> gint;
     std::vector<cd> gcoef;
+    std::vector<int64_t> gsrc;
     std::vector<int32_t> slot((size_t)n, -1);
     int ngroups = 0;
     for (int64_t r = 0; r < n; ++r) {
@@ -583,6 +611,7 @@ static bool build_csc_groups(const RowLists &Xcols, int64_t xrows, qob_dtype dt,
         const size_t gi = gint.size();
         gint.resize(gi + NI, 0);
         gcoef.resize((size_t)(ngroups + 1) * GM * GM, cd(0, 0));
+        gsrc.resize(gcoef.size(), -1);
         gint[gi] = sz;
         gint[gi + 1] = (int32_t)in.size();
         uint64_t mask = 0;
@@ -594,6 +623,7 @@ static bool build_csc_groups(const RowLists &Xcols, int64_t xrows, qob_dtype dt,
                 if (mask & bit) return false;  // repeated entry: leave it to the CSC kernel
                 mask |= bit;
                 gcoef[(size_t)ngroups * GM * GM + (size_t)o * GM + q] = Xcols.val[p];
+                gsrc[(size_t)ngroups * GM * GM + (size_t)o * GM + q] = p;
             }
         }
         for (size_t q = 0; q < in.size(); ++q) gint[gi + 4 + GM + q] = in[q];
@@ -603,6 +633,7 @@ static bool build_csc_groups(const RowLists &Xcols, int64_t xrows, qob_dtype dt,
     }
     if (C.gint.upload(gint.data(), gint.size() * 4) != QOB_OK || upload_vals(C.gcoef, gcoef, dt) != QOB_OK)
         return false;
+    C.gsrc = std::move(gsrc);
     C.dev.gint = (const int32_t *)C.gint.p;
     C.dev.gcoef = C.gcoef.p;
     C.dev.ngroups = ngroups;
@@ -631,20 +662,28 @@ static qob_status build_csc(const RowLists &Xcols, int64_t xrows, qob_dtype dt, 
     return QOB_OK;
 }
 
-static qob_status sparse_sell(SparseData &S, int v, qob_dtype dt, SellDev &out) {
+static qob_status refresh_sell(SparseData &S, int v, SellTab &t, qob_dtype dt, hipStream_t s);
+static qob_status refresh_csc(SparseData &S, int v, CscTab &t, qob_dtype dt, hipStream_t s);
+
+// A table of a materialised sum whose coefficients are held on the device
+// takes its values from them as soon as it is built (on stream s).
+static bool mat_device_coefs(const SparseData &S) { return S.mc && S.mc->dcoef_on && *S.mc->dcoef_on; }
+
+static qob_status sparse_sell(SparseData &S, int v, qob_dtype dt, SellDev &out, hipStream_t s = nullptr) {
     std::lock_guard<std::mutex> g(S.mu);
     const int key = v * 2 + (int)dt;
     auto it = S.sell.find(key);
     if (it == S.sell.end()) {
         auto t = std::make_shared<SellTab>();
         QOB_TRY(build_sell(S.rows(v), dt, *t));
+        if (mat_device_coefs(S)) QOB_TRY(refresh_sell(S, v, *t, dt, s));
         it = S.sell.emplace(key, t).first;
     }
     out = it->second->dev;
     return QOB_OK;
 }
 
-static qob_status sparse_csc(SparseData &S, int v, qob_dtype dt, CscDev &out) {
+static qob_status sparse_csc(SparseData &S, int v, qob_dtype dt, CscDev &out, hipStream_t s = nullptr) {
     std::lock_guard<std::mutex> g(S.mu);
     const int key = v * 2 + (int)dt;
     auto it = S.cols.find(key);
@@ -654,6 +693,7 @@ static qob_status sparse_csc(SparseData &S, int v, qob_dtype dt, CscDev &out) {
         const RowLists cl = S.rows(v ^ V_T);
         const int64_t xr = (v & 1) ? S.n : S.m;
         QOB_TRY(build_csc(cl, xr, dt, *t));
+        if (mat_device_coefs(S)) QOB_TRY(refresh_csc(S, v, *t, dt, s));
         it = S.cols.emplace(key, t).first;
     }
     out = it->second->dev;
@@ -763,6 +803,84 @@ static qob_status run_refresh(const RefreshMap &R, qob_dtype dt, const void *coe
                                   (const double2 *)R.bval.p, (const int32_t *)R.dst.p, (const double2 *)coef, vals,
                                   (int32_t *)prog, s);
     if (e != hipSuccess) return fail(QOB_EHIP, "coefficient refresh: %s", hipGetErrorString(e));
+    return QOB_OK;
+}
+
+// Refresh map of a table of a materialised sum: slot sl holds entry src[sl]
+// of the CSR (csc_space false) or CSC lists of the sum, i.e. the entry's
+// coefficient-weighted contributions.  Tables of conjugated variants are not
+// refreshable (the coefficients would need conjugating).
+static qob_status mat_refresh_map(const MatContrib &mc, bool csc_space, const std::vector<int64_t> &src,
+                                  RefreshMap &R) {
+    if (src.size() >= (size_t)INT32_MAX) return fail(QOB_EUNSUPPORTED, "table too large to refresh");
+    std::vector<int32_t> ptr(1, 0), term;
+    std::vector<double2> bval;
+    for (int64_t q : src) {
+        if (q >= 0) {
+            const int64_t e = csc_space ? mc.csc2csr[(size_t)q] : q;
+            for (int64_t c = mc.cptr[(size_t)e]; c < mc.cptr[(size_t)e + 1]; ++c) {
+                term.push_back(mc.cterm[(size_t)c]);
+                bval.push_back(d2(mc.cbase[(size_t)c]));
+            }
+        }
+        if (term.size() >= (size_t)INT32_MAX) return fail(QOB_EUNSUPPORTED, "table too large to refresh");
+        ptr.push_back((int32_t)term.size());
+    }
+    if (term.empty()) {
+        term.push_back(0);
+        bval.push_back(make_double2(0, 0));
+    }
+    R.nslots = (int)src.size();
+    R.nvals = (int)src.size();
+    QOB_TRY(R.ptr.upload(ptr.data(), ptr.size() * 4));
+    QOB_TRY(R.term.upload(term.data(), term.size() * 4));
+    QOB_TRY(R.bval.upload(bval.data(), bval.size() * sizeof(double2)));
+    const int32_t zero = 0;
+    QOB_TRY(R.dst.upload(&zero, 4));
+    return QOB_OK;
+}
+
+// Values of a materialised sum's table from the owner's device coefficients
+// (the table was built from rows(v) / rows(v ^ V_T) of the sum).
+static qob_status refresh_sell(SparseData &S, int v, SellTab &t, qob_dtype dt, hipStream_t s) {
+    if (v & V_CJ) return fail(QOB_EUNSUPPORTED, "conjugated table of a sum with device-held coefficients");
+    if (!t.rmap) {
+        auto R = std::make_shared<RefreshMap>();
+        QOB_TRY(mat_refresh_map(*S.mc, (v & V_T) != 0, t.src, *R));
+        t.rmap = R;
+    }
+    return t.src.empty() ? QOB_OK : run_refresh(*t.rmap, dt, *S.mc->dcoef, t.val.p, nullptr, s);
+}
+
+static qob_status refresh_csc(SparseData &S, int v, CscTab &t, qob_dtype dt, hipStream_t s) {
+    if (v & V_CJ) return fail(QOB_EUNSUPPORTED, "conjugated table of a sum with device-held coefficients");
+    const bool csc_space = ((v ^ V_T) & V_T) != 0;
+    if (!t.rmap) {
+        // val holds the row lists' values in order
+        std::vector<int64_t> id(S.csr.idx.size());
+        for (size_t q = 0; q < id.size(); ++q) id[q] = (int64_t)q;
+        auto R = std::make_shared<RefreshMap>();
+        QOB_TRY(mat_refresh_map(*S.mc, csc_space, id, *R));
+        t.rmap = R;
+    }
+    if (t.dev.colptr && !S.csr.idx.empty()) QOB_TRY(run_refresh(*t.rmap, dt, *S.mc->dcoef, t.val.p, nullptr, s));
+    if (t.dev.gcoef && !t.gsrc.empty()) {
+        if (!t.grmap) {
+            auto R = std::make_shared<RefreshMap>();
+            QOB_TRY(mat_refresh_map(*S.mc, csc_space, t.gsrc, *R));
+            t.grmap = R;
+        }
+        QOB_TRY(run_refresh(*t.grmap, dt, *S.mc->dcoef, t.gcoef.p, nullptr, s));
+    }
+    return QOB_OK;
+}
+
+// every table of a materialised sum from the device coefficients (after
+// they change)
+static qob_status refresh_mat(SparseData &S, hipStream_t s) {
+    std::lock_guard<std::mutex> g(S.mu);
+    for (auto &kv : S.sell) QOB_TRY(refresh_sell(S, kv.first / 2, *kv.second, (qob_dtype)(kv.first % 2), s));
+    for (auto &kv : S.cols) QOB_TRY(refresh_csc(S, kv.first / 2, *kv.second, (qob_dtype)(kv.first % 2), s));
     return QOB_OK;
 }
 
@@ -1599,7 +1717,7 @@ static qob_status apply_sparse(const Apply &ap, OpNode &op, const void *in, void
     const int v = op.var ^ (ap.trans ? V_T : V_N);
     if (ap.lead == 1) {
         SellDev X;
-        QOB_TRY(sparse_sell(*op.sp, v, ap.dt, X));
+        QOB_TRY(sparse_sell(*op.sp, v, ap.dt, X, ap.s));
         hipError_t e = launch_sell_left(ap.dt, X, in, ap.trail, out, d2(alpha), d2(beta), ap.s);
         if (e != hipSuccess) return fail(QOB_EHIP, "sparse left launch: %s", hipGetErrorString(e));
     } else {
@@ -1607,7 +1725,7 @@ static qob_status apply_sparse(const Apply &ap, OpNode &op, const void *in, void
         // X = op_v along the column index: out[:, c] = sum_r X[c, r] in[:, r]
         // i.e. R = B * X^T  -> CSC of X^T = rows of X
         CscDev C;
-        QOB_TRY(sparse_csc(*op.sp, v ^ V_T, ap.dt, C));
+        QOB_TRY(sparse_csc(*op.sp, v ^ V_T, ap.dt, C, ap.s));
         hipError_t e = launch_csc_right(ap.dt, C, in, ap.lead, out, d2(alpha), d2(beta), ap.s);
         if (e != hipSuccess) return fail(QOB_EHIP, "sparse right launch: %s", hipGetErrorString(e));
     }
@@ -1858,7 +1976,8 @@ static qob_status apply_qubit(const Apply &ap, OpNode &op, const void *in, void 
 
 extern int g_ls_materialize;
 static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const std::vector<cd> &lcoef,
-                           const std::vector<size_t> &sp_terms, std::shared_ptr<SparseData> &out);
+                           const std::vector<size_t> &lt_terms, const std::vector<size_t> &sp_terms,
+                           std::shared_ptr<SparseData> &out);
 
 static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *out, cd alpha, cd beta) {
     const int64_t dout = op_out_dim(op, ap.trans);
@@ -1870,7 +1989,7 @@ static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *ou
     if (lt_pure_sparse(op) && g_ls_materialize && nvec >= 8 && op.m < (1ll << 31) && op.n < (1ll << 31)) {
         // on a density matrix: the tensor's sparse matrix (see ls_materialize)
         std::shared_ptr<SparseData> SD;
-        if (ls_materialize(op, {&op}, {cd(1, 0)}, {}, SD)) {
+        if (ls_materialize(op, {&op}, {cd(1, 0)}, {0}, {}, SD)) {
             OpNode tmp;
             tmp.kind = K_SPARSE;
             tmp.m = op.m;
@@ -1903,7 +2022,8 @@ static qob_status apply_lt(const Apply &ap, OpNode &op, const void *in, void *ou
 int g_ls_materialize = 1;
 
 static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const std::vector<cd> &lcoef,
-                           const std::vector<size_t> &sp_terms, std::shared_ptr<SparseData> &out) {
+                           const std::vector<size_t> &lt_terms, const std::vector<size_t> &sp_terms,
+                           std::shared_ptr<SparseData> &out) {
     std::lock_guard<std::mutex> g(op.mu);
     const std::string key = "mat" + std::to_string(op.version);
     auto it = op.sums.find(key);
@@ -1925,7 +2045,7 @@ static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const s
             return false;
         }
         TermSpec ts;
-        ts.coef = lcoef[t] * lt.factor;
+        ts.coef = lt.factor;  // the coefficient lcoef[t] is applied when entries are summed
         for (size_t i = 0; i < lt.fop.size(); ++i)
             if (lt.fop[i]->kind != K_EYE) store[t].push_back(factor_rows(*lt.fop[i], false));
         size_t k2 = 0;
@@ -1963,7 +2083,17 @@ static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const s
     S->csr.nc = n;
     S->csr.ptr.assign((size_t)m + 1, 0);
     std::vector<int64_t> dig(ns);
-    std::vector<std::pair<int64_t, cd>> row;
+    // (column, term, value without the term's coefficient)
+    struct Contrib {
+        int64_t k;
+        int32_t t;
+        cd b;
+    };
+    std::vector<Contrib> row;
+    auto MC = std::make_shared<MatContrib>();
+    MC->cptr.push_back(0);
+    // coefficient of term t (a single LazyTensor: one term of coefficient 1)
+    const std::vector<cd> cf = op.kind == K_LS ? op.coef : std::vector<cd>(1, cd(1, 0));
     for (int64_t i = 0; i < m; ++i) {
         row.clear();
         int64_t rem = i;
@@ -1971,11 +2101,13 @@ static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const s
             dig[site] = rem / stl[site];
             rem %= stl[site];
         }
-        for (const TermSpec &ts : specs) {
+        for (size_t st = 0; st < specs.size(); ++st) {
+            const TermSpec &ts = specs[st];
+            const int32_t term = (int32_t)lt_terms[st];
             // depth-first over sites: column offset and value so far
             std::function<void(int, int64_t, cd)> rec = [&](int site, int64_t k, cd v) {
                 if (site == ns) {
-                    row.emplace_back(k, v);
+                    row.push_back({k, term, v});
                     return;
                 }
                 auto f = std::find(ts.sites.begin(), ts.sites.end(), site);
@@ -1991,21 +2123,34 @@ static bool ls_materialize(OpNode &op, const std::vector<OpNode *> &lts, const s
         }
         for (size_t q = 0; q < sp_terms.size(); ++q) {
             const RowLists &R = sprows[q];
-            const cd c = op.coef[sp_terms[q]];
-            for (int32_t p = R.ptr[i]; p < R.ptr[i + 1]; ++p) row.emplace_back(R.idx[p], c * R.val[p]);
+            for (int32_t p = R.ptr[i]; p < R.ptr[i + 1]; ++p) row.push_back({R.idx[p], (int32_t)sp_terms[q], R.val[p]});
         }
-        std::stable_sort(row.begin(), row.end(),
-                         [](const std::pair<int64_t, cd> &a, const std::pair<int64_t, cd> &b) { return a.first < b.first; });
+        std::stable_sort(row.begin(), row.end(), [](const Contrib &a, const Contrib &b) { return a.k < b.k; });
         for (size_t q = 0; q < row.size();) {
-            int64_t k = row[q].first;
+            int64_t k = row[q].k;
             cd v = 0;
-            for (; q < row.size() && row[q].first == k; ++q) v += row[q].second;
+            for (; q < row.size() && row[q].k == k; ++q) {
+                v += cf[(size_t)row[q].t] * row[q].b;
+                MC->cterm.push_back(row[q].t);
+                MC->cbase.push_back(row[q].b);
+            }
             S->csr.idx.push_back((int32_t)k);
             S->csr.val.push_back(v);
+            MC->cptr.push_back((int64_t)MC->cterm.size());
         }
         S->csr.ptr[i + 1] = (int32_t)S->csr.idx.size();
     }
     S->csc = transpose_rows(S->csr, false);
+    // CSC entry -> CSR entry (the transpose visits rows in order)
+    {
+        MC->csc2csr.resize(S->csr.idx.size());
+        std::vector<int32_t> pos(S->csc.ptr.begin(), S->csc.ptr.end() - 1);
+        for (int64_t r = 0; r < m; ++r)
+            for (int32_t p = S->csr.ptr[r]; p < S->csr.ptr[r + 1]; ++p) MC->csc2csr[pos[S->csr.idx[p]]++] = p;
+    }
+    MC->dcoef = &op.dcoef.p;
+    MC->dcoef_on = &op.dcoef_on;
+    S->mc = MC;
     op.sums.emplace(key, S);
     out = S;
     return true;
@@ -2034,12 +2179,13 @@ static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *ou
     }
     std::vector<OpNode *> lts;
     std::vector<cd> lcoef;
-    std::vector<size_t> sp_terms, rest;
+    std::vector<size_t> lt_terms, sp_terms, rest;
     for (size_t t = 0; t < op.terms.size(); ++t) {
         OpNode &x = *op.terms[t];
         if (lt_pure_sparse(x)) {
             lts.push_back(&x);
             lcoef.push_back(op.coef[t]);
+            lt_terms.push_back(t);
         } else if (x.kind == K_SPARSE) {
             sp_terms.push_back(t);
         } else {
@@ -2048,9 +2194,11 @@ static qob_status apply_ls(const Apply &ap, OpNode &op, const void *in, void *ou
     }
     cd b = beta;
     const int64_t nvec = ap.lead == 1 ? ap.trail : ap.lead;
-    if (!lts.empty() && g_ls_materialize && !op.dcoef_on && nvec >= 8 && op.m < (1ll << 31) && op.n < (1ll << 31)) {
+    if (!lts.empty() && g_ls_materialize && nvec >= 8 && op.m < (1ll << 31) && op.n < (1ll << 31)) {
+        // with device-held coefficients the tables take their values from
+        // them (refreshed on the device whenever they change)
         std::shared_ptr<SparseData> SD;
-        if (ls_materialize(op, lts, lcoef, sp_terms, SD)) {
+        if (ls_materialize(op, lts, lcoef, lt_terms, sp_terms, SD)) {
             OpNode tmp;
             tmp.kind = K_SPARSE;
             tmp.m = op.m;
@@ -2588,6 +2736,8 @@ qob_status qob_lazysum_set_factors_d(qob_ctx ctx, qob_op sum, const void *dfacto
         QOB_TRY(run_refresh(*P.rmap, P.dt, n.dcoef.p, P.vals.p, nullptr, st));
     }
     n.dcoef_on = true;
+    for (auto &kv : n.sums)
+        if (kv.second && kv.second->mc) QOB_TRY(refresh_mat(*kv.second, st));
     return QOB_OK;
 }
 
@@ -2830,11 +2980,12 @@ qob_status qob_tune_set(const char *name, int64_t value) {
         return fail(QOB_EARG, "fused_gpw must be 0 (automatic), 1, 2 or 4");
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
     if (!strcmp(name, "qubit_remap") && value > 2) return fail(QOB_EARG, "qubit_remap must be 0, 1 or 2");
+    if (!strcmp(name, "qubit_ntfar") && value > 8) return fail(QOB_EARG, "qubit_ntfar must be in 0..8");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "sell_cline") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
          !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles") ||
-         !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg") || !strcmp(name, "qubit_ntfar")) &&
+         !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 0 && value != 256 && value != 512 && value != 1024)

@@ -373,8 +373,10 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
     const ItemWalk wk = qb_walk(flags, xs, nitems);
     // with flags & 16, partner tiles at least as many tiles away as an XCD
     // has workgroups lie outside the run of tiles in flight there: read once
+    // (flags >> 5: the distance is that run's length halved that many times)
     const uint32_t far_tiles =
-        !(flags & 16) ? 0xffffffffu : (flags & 8) ? (uint32_t)wk.step : max(1u, (uint32_t)gridDim.x >> 3);
+        !(flags & 16) ? 0xffffffffu
+                      : max(1u, ((flags & 8) ? (uint32_t)wk.step : (uint32_t)gridDim.x >> 3) >> ((flags >> 5) & 7));
     for (int it = wk.it; it < wk.end; it += wk.step) {
         QubitItemDev w;
         QubitStateDev S;
@@ -583,8 +585,9 @@ static int qubit_nth(qob_dtype dt) { return g_qubit_nth ? g_qubit_nth : (dt == Q
 // item range for the whole launch (a batch's states are assigned to XCDs
 // whole; the columns form splits its items into 8 contiguous ranges).
 int g_qubit_remap = 2;
-// 1: partner tiles of flips whose tile distance is at least the number of
-// workgroups walking the XCD's range are read with non-temporal loads
+// k >= 1: partner tiles of flips whose tile distance is at least the number
+// of workgroups walking the XCD's range, halved k - 1 times, are read with
+// non-temporal loads; 0: none
 int g_qubit_ntfar = 1;
 
 // flags bits 4 / 8 and the item ranges of a launch with g workgroups
@@ -651,7 +654,7 @@ static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, co
     const unsigned g = qubit_grid(dt, nitems);
     QubitSplit ws;
     const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (gamma.x == 0.0 && gamma.y == 0.0 ? 2 : 0) |
-                      qubit_walk_flags(nitems, g, xs, ws) | (g_qubit_ntfar ? 16 : 0);
+                      qubit_walk_flags(nitems, g, xs, ws) | (g_qubit_ntfar ? 16 | ((g_qubit_ntfar - 1) << 5) : 0);
 #define QOB_QB(TT, NT, TB)                                                                                    \
     do {                                                                                                      \
         auto kfn = k_qubit<TT, NT, TB>;                                                                       \

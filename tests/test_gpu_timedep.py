@@ -164,3 +164,85 @@ def test_graph_replayed_time_steps_match_eager(qubit):
             assert O.rel_frobenius(got, xe.to_numpy()) < 1e-14, rep
     finally:
         _lib.call("qob_tune_set", b"qubit_enable", 1)
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 1e-5)])
+def test_density_matrix_steps_with_device_coefficients(side, dtype, tol):
+    """rho <- rho + c H(t_k) rho (or rho H(t_k)) with coefficients from a
+    device table row per step.  On a density matrix an all-sparse sum runs
+    as its summed sparse matrix, whose tables take their values from the
+    device coefficients (refreshed on the device at each set_time_row): a
+    captured graph of the steps is replayed three times against eager steps
+    with host set_time!, against the oracle, and against the same steps with
+    the summed matrix turned off (the one-launch gather)."""
+    import ctypes as C
+    spec = _chain(7)
+    ops, oracle_at = _pieces(spec, COEFS)
+    D = len(spec.basis)
+    K, dt = 6, 0.07
+    times = [0.1 + j * dt for j in range(K)]
+    c = -1j * dt
+    g = np.random.default_rng(17)
+    rho0 = g.standard_normal((D, D)) + 1j * g.standard_normal((D, D))
+    want = rho0.copy()
+    for t in times:
+        kk = np.zeros((D, D), complex)
+        if side == "left":
+            O.mul_op_left(kk, oracle_at(t), want, 1, 0)
+        else:
+            O.mul_op_right(kk, want, oracle_at(t), 1, 0)
+        want = want + c * kk
+    qdt = Q.QOB_C64 if dtype == np.complex128 else Q.QOB_C32
+    esz = 16 if dtype == np.complex128 else 8
+    _lib.call("qob_tune_set", b"qubit_enable", 0)
+    results = {}
+    try:
+        for mat in (1, 0):
+            _lib.call("qob_tune_set", b"ls_materialize", mat)
+            H = Q.TimeDependentSum(COEFS, ops, init_time=0.0)
+            ctx = Q.context()
+            st = ctx.stream
+            table = H.coefficient_table(times)
+            xg, kg = Q.DeviceArray.from_numpy(rho0, dtype), Q.DeviceArray(D * D, dtype, (D, D))
+            x, k = Q.Operator(spec.basis, spec.basis, xg), Q.Operator(spec.basis, spec.basis, kg)
+
+            def step(j):
+                H.set_time_row(table, j, st)
+                if side == "left":
+                    Q.mul_(k, H, x)
+                else:
+                    Q.mul_(k, x, H)
+                _lib.call("qob_axpy", ctx.h, qdt, D * D, _lib.c64(c), kg.ptr, xg.ptr, st)
+
+            step(0)  # tables and coefficient maps are built outside capture
+            ctx.synchronize()
+            with Q.Graph(st) as gr:
+                for j in range(K):
+                    step(j)
+            host = np.asfortranarray(rho0.astype(dtype))
+            for rep in range(3):
+                _lib.call("qob_memcpy_htod", ctx.h, xg.ptr, host.ctypes.data_as(C.c_void_p), esz * D * D, None)
+                gr.launch()
+                ctx.synchronize()
+                got = xg.to_numpy()
+                assert O.rel_frobenius(got, want) < tol, (mat, rep)
+            results[mat] = got
+            del gr
+            if mat == 1:
+                # eager steps with host set_time! on an independent sum
+                Hh = Q.TimeDependentSum(COEFS, ops, init_time=0.0)
+                xe = Q.Operator(spec.basis, spec.basis, rho0, dtype)
+                ke = Q.Operator(spec.basis, spec.basis, np.zeros((D, D)), dtype)
+                for t in times:
+                    Hh.set_time_(t)
+                    if side == "left":
+                        Q.mul_(ke, Hh, xe)
+                    else:
+                        Q.mul_(ke, xe, Hh)
+                    Q.axpy_(xe, c, ke)
+                assert O.rel_frobenius(xe.to_numpy(), want) < tol
+        assert O.rel_frobenius(results[1], results[0]) < tol
+    finally:
+        _lib.call("qob_tune_set", b"ls_materialize", 1)
+        _lib.call("qob_tune_set", b"qubit_enable", 1)
