@@ -34,6 +34,11 @@ namespace qob {
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
 extern int g_qubit_nth, g_qubit_grid, g_qubit_remap, g_qubit_ntfar;
+// order of a run's tiles in a batch (batches created after it is set):
+// 0 index order; 1 binary-reflected Gray code of the index, so that tiles
+// walked in turn differ in one bit and the partner runs of the high bits
+// follow each other; 2 the same on the index bits above the low 6 only
+int g_qubit_order = 1;
 extern int g_mode_kernel, g_fused_gpw, g_csc_rows, g_sell_unr, g_csc_order, g_fused_tiles, g_fused_resident;
 extern int g_fused_reg;
 extern int g_sell_align, g_csc_waves, g_csc_sweep, g_sell_cjchunk, g_csc_gkernel, g_csc_gremap, g_sell_cremap;
@@ -1602,14 +1607,15 @@ static bool qubit_terms(OpNode &op, bool trans, std::vector<TermSpec> &specs, st
 
 int g_qubit_enable = 1;
 extern int g_qubit_tbits;
+int qubit_tbits(qob_dtype dt);
 
 static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s, std::shared_ptr<QubitPlan> &out,
                              bool nosplit = false) {
     std::lock_guard<std::mutex> g(op.mu);
-    const int tb = g_qubit_tbits;
+    const int tbd = qubit_tbits(dt);
     const int split = nosplit ? 0 : g_qubit_split;
     const std::string k = "qb:" + std::to_string(op.version) + (trans ? ":T:" : ":N:") + std::to_string((int)dt) +
-                          ":" + std::to_string(tb) + ":" + std::to_string(split);
+                          ":" + std::to_string(tbd) + ":" + std::to_string(split);
     auto it = op.qplans.find(k);
     if (it == op.qplans.end()) {
         auto P = std::make_shared<QubitPlan>();
@@ -1618,7 +1624,11 @@ static qob_status qubit_plan(OpNode &op, bool trans, qob_dtype dt, hipStream_t s
         std::vector<cd> base;
         int nbits = 0;
         QubitProgHost H;
-        if (qubit_terms(op, trans, specs, store, nbits, &base) && compile_qubit(nbits, specs, H, tb, split)) {
+        // a state smaller than the tile gets a tile of its own size, at least
+        // 2^11 (one workgroup with half or more of its lanes busy)
+        const bool terms = qubit_terms(op, trans, specs, store, nbits, &base);
+        const int tb = nbits < tbd ? std::max(std::min(tbd, 11), nbits) : tbd;
+        if (terms && compile_qubit(nbits, specs, H, tb, split)) {
             std::vector<int32_t> all = H.prog;
             if (!H.prog_hi.empty()) {
                 P->hi_off = (int32_t)all.size();
@@ -2943,7 +2953,7 @@ static int *tune_slot(const char *name) {
                {"qubit_tbits", &g_qubit_tbits},   {"qubit_nth", &g_qubit_nth},
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
                {"qubit_split", &g_qubit_split},   {"mode_kernel", &g_mode_kernel}, {"fused_reg", &g_fused_reg},
-               {"qubit_ntfar", &g_qubit_ntfar},
+               {"qubit_ntfar", &g_qubit_ntfar}, {"qubit_order", &g_qubit_order},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
@@ -2965,7 +2975,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
         return fail(QOB_EARG, "csc_rb must be 64, 128 or 256");
     if (!strcmp(name, "sell_cj") && value != 1 && value != 2 && value != 4 && value != 8)
         return fail(QOB_EARG, "sell_cj must be 1, 2, 4 or 8");
-    if (!strcmp(name, "qubit_tbits") && (value < 10 || value > 13)) return fail(QOB_EARG, "qubit_tbits must be in 10..13");
+    if (!strcmp(name, "qubit_tbits") && value != 0 && (value < 10 || value > 13))
+        return fail(QOB_EARG, "qubit_tbits must be 0 (by element type) or in 10..13");
     if (!strcmp(name, "mode_kernel") && value > 2)
         return fail(QOB_EARG, "mode_kernel must be 0 (automatic), 1 (tiled GEMM for every dense mode product) or 2 "
                               "(fibre-group kernel for leading dimensions below 16 as well)");
@@ -2981,6 +2992,7 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_split") && value > 1) return fail(QOB_EARG, "qubit_split must be 0 or 1");
     if (!strcmp(name, "qubit_remap") && value > 2) return fail(QOB_EARG, "qubit_remap must be 0, 1 or 2");
     if (!strcmp(name, "qubit_ntfar") && value > 8) return fail(QOB_EARG, "qubit_ntfar must be in 0..8");
+    if (!strcmp(name, "qubit_order") && value > 2) return fail(QOB_EARG, "qubit_order must be 0, 1 or 2");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
          !strcmp(name, "sell_cline") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
@@ -3362,7 +3374,19 @@ qob_status qob_batch_create(qob_ctx ctx, int n, const qob_op *ops, qob_dtype dt,
     B->ctx = ctx;
     B->dt = dt;
     B->n = n;
-    B->tbits = g_qubit_tbits;
+    B->tbits = qubit_tbits(dt);
+    if (!g_qubit_tbits && B->tbits > 11) {
+        // tiles larger than 2^11 only when most of the elements are in states
+        // at least that large (a smaller state leaves lanes of its tile idle)
+        int64_t big = 0, all = 0;
+        for (int j = 0; j < n; ++j) {
+            if (!ops[j]) return fail(QOB_EARG, "null operator %d", j);
+            const int64_t d = ops[j]->node->n;
+            all += d;
+            if (d >= ((int64_t)1 << B->tbits)) big += d;
+        }
+        if (2 * big < all) B->tbits = 11;
+    }
     std::vector<int32_t> prog;
     std::vector<cd> vals;
     for (int j = 0; j < n; ++j) {
@@ -3412,7 +3436,13 @@ qob_status qob_batch_create(qob_ctx ctx, int n, const qob_op *ops, qob_dtype dt,
         for (int64_t t0 = 0; t0 < nt; t0 += run) {
             const int x = (int)(std::min_element(load, load + 8) - load);
             load[x] += run << tb;
-            for (int64_t t = t0; t < t0 + run; ++t) {
+            for (int64_t k = 0; k < run; ++k) {
+                // run is a power of two and t0 a multiple of it: any order of
+                // the index bits below run stays inside the run
+                const int64_t q = g_qubit_order == 1   ? k ^ (k >> 1)
+                                  : g_qubit_order == 2 ? (((k >> 6) ^ (k >> 7)) << 6) | (k & 63)
+                                                       : k;
+                const int64_t t = t0 + q;
                 xl[x].push_back({j, (int32_t)t});
                 if (B->prog_hi_off[j] >= 0) xl_hi[x].push_back({j, (int32_t)t});
             }

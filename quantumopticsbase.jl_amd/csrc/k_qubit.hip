@@ -575,11 +575,14 @@ __global__ void __launch_bounds__(NTH) k_qubit_axpy(const QubitStateDev *__restr
 
 }  // namespace
 
-// threads per workgroup; 0: 512 for ComplexF64, 256 for ComplexF32 (measured
-// fastest on the 256-chain batch: 4 elements per lane in ComplexF64, 8 in
-// ComplexF32 at 2^11-element tiles)
+// threads per workgroup; 0: by element type and tile, 4 elements per lane in
+// ComplexF64 (1024 threads for its 2^12-element tiles) and 8 in ComplexF32
+// (256 threads for 2^11), measured fastest on the 256-chain batch
 int g_qubit_nth = 0;
-static int qubit_nth(qob_dtype dt) { return g_qubit_nth ? g_qubit_nth : (dt == QOB_C64 ? 512 : 256); }
+static int qubit_nth(qob_dtype dt, int tbits) {
+    if (g_qubit_nth) return g_qubit_nth;
+    return std::min(1024, std::max(256, (1 << tbits) >> (dt == QOB_C64 ? 2 : 3)));
+}
 // 0: workgroup b walks items b, b + G, ...; 1: the same rounds of G items with
 // each XCD given a contiguous run of every round; 2: each XCD walks its own
 // item range for the whole launch (a batch's states are assigned to XCDs
@@ -604,21 +607,27 @@ static int qubit_walk_flags(int nitems, unsigned g, const QubitSplit *xs, QubitS
     return g_qubit_remap ? 4 : 0;
 }
 // 0: one workgroup per item when the items fit in 5 workgroups per CU (many small states: no
-// persistent loop, every item starts at once); otherwise a persistent grid of 2 workgroups per CU
-// for ComplexF64 and 4 for ComplexF32, the most that fit by registers (84 and 54 VGPRs at 512
-// threads). Each wave keeps two flip groups' partner loads in flight; 2^11-element tiles keep
-// 2-3 MB of tiles in flight per XCD, inside its 4 MB L2.
+// persistent loop, every item starts at once); otherwise a persistent grid of 1024 / nth
+// workgroups per CU (16 waves), the most that fit by registers (84 VGPRs in ComplexF64, 54 in
+// ComplexF32 at 4 and 8 elements per lane). Each wave keeps two flip groups' partner loads in
+// flight; the tiles in flight per XCD (2 MB in ComplexF64: 32 of 2^12 elements) stay well
+// inside its 4 MB L2, which must also hold the partner tiles read again.
 int g_qubit_grid = 0;
 
-static unsigned qubit_grid(qob_dtype dt, int nitems) {
+static unsigned qubit_grid(int nth, int nitems) {
     const int ncu = device_cu_count();
     const int64_t g = g_qubit_grid > 0           ? g_qubit_grid
                       : nitems <= (int64_t)ncu * 5 ? nitems
-                                                   : (int64_t)ncu * (dt == QOB_C64 ? 2 : 4);
+                                                   : (int64_t)ncu * std::max(1, 1024 / nth);
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, nitems));
 }
 
-int g_qubit_tbits = 11;
+// log2 of the tile; 0: 12 for ComplexF64 (one partner-tile read per element
+// fewer than 2^11: 3.1 reads per element instead of 4.1 on the 256-chain
+// batch, 10% faster), 11 for ComplexF32 (bound by issue, not traffic: larger
+// tiles measured slower)
+int g_qubit_tbits = 0;
+int qubit_tbits(qob_dtype dt) { return g_qubit_tbits ? g_qubit_tbits : (dt == QOB_C64 ? 12 : 11); }
 
 static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
                                    int tbits, const int32_t *prog, const void *vals, double2 alpha, double2 beta,
@@ -651,7 +660,8 @@ static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, co
     if (tbits > 13) tbits = 13;
     const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const unsigned g = qubit_grid(dt, nitems);
+    const int nth = qubit_nth(dt, tbits);
+    const unsigned g = qubit_grid(nth, nitems);
     QubitSplit ws;
     const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (gamma.x == 0.0 && gamma.y == 0.0 ? 2 : 0) |
                       qubit_walk_flags(nitems, g, xs, ws) | (g_qubit_ntfar ? 16 | ((g_qubit_ntfar - 1) << 5) : 0);
@@ -671,8 +681,8 @@ static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, co
     } while (0)
 #define QOB_QB_NT(TT, TB)                                  \
     do {                                                   \
-        if (qubit_nth(dt) == 512) QOB_QB(TT, 512, TB);       \
-        else if (qubit_nth(dt) == 1024) QOB_QB(TT, 1024, TB); \
+        if (nth == 512) QOB_QB(TT, 512, TB);                 \
+        else if (nth == 1024) QOB_QB(TT, 1024, TB);          \
         else QOB_QB(TT, 256, TB);                          \
     } while (0)
 #define QOB_QB_TB(TT)                          \
@@ -697,7 +707,8 @@ static hipError_t launch_qubit_hi_any(qob_dtype dt, const QubitStateDev *states,
     if (nitems == 0) return hipSuccess;
     if (tbits < 10 || tbits > 13) return hipErrorInvalidValue;
     const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
-    const unsigned g = qubit_grid(dt, nitems);
+    const int nth = qubit_nth(dt, tbits);
+    const unsigned g = qubit_grid(nth, nitems);
     QubitSplit ws;
     const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | qubit_walk_flags(nitems, g, xs, ws);
 #define QOB_QH(TT, NT, TB)                                                                                    \
@@ -716,8 +727,8 @@ static hipError_t launch_qubit_hi_any(qob_dtype dt, const QubitStateDev *states,
     } while (0)
 #define QOB_QH_NT(TT, TB)                                  \
     do {                                                   \
-        if (qubit_nth(dt) == 512) QOB_QH(TT, 512, TB);       \
-        else if (qubit_nth(dt) == 1024) QOB_QH(TT, 1024, TB); \
+        if (nth == 512) QOB_QH(TT, 512, TB);                 \
+        else if (nth == 1024) QOB_QH(TT, 1024, TB);          \
         else QOB_QH(TT, 256, TB);                          \
     } while (0)
 #define QOB_QH_TB(TT)                            \
@@ -764,7 +775,8 @@ hipError_t launch_qubit_dot(qob_dtype dt, QubitStateDev single, const void *u, i
     const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
 #define QOB_QD(TT, TB)                                                                                          \
     do {                                                                                                        \
-        auto kfn = k_qubit_dot<TT, 512, TB>;                                                                    \
+        constexpr int NTD = TB >= 12 ? 1024 : 512; /* 4 elements per lane up to 2^12-element tiles */          \
+        auto kfn = k_qubit_dot<TT, NTD, TB>;                                                                    \
         static std::atomic<int> lds_set{0};                                                                     \
         if (!lds_set.load()) {                                                                                  \
             hipError_t e_ = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,  \
@@ -772,7 +784,7 @@ hipError_t launch_qubit_dot(qob_dtype dt, QubitStateDev single, const void *u, i
             if (e_ != hipSuccess) return e_;                                                                    \
             lds_set.store(1);                                                                                   \
         }                                                                                                       \
-        hipLaunchKernelGGL(kfn, dim3(g), dim3(512), lds, s, single, (const TT *)u, (int)ntiles, prog,           \
+        hipLaunchKernelGGL(kfn, dim3(g), dim3(NTD), lds, s, single, (const TT *)u, (int)ntiles, prog,           \
                            (const TT *)vals, partials);                                                         \
         return hipGetLastError();                                                                               \
     } while (0)
@@ -793,7 +805,7 @@ hipError_t launch_qubit_dot(qob_dtype dt, QubitStateDev single, const void *u, i
 hipError_t launch_qubit_axpy(qob_dtype dt, const QubitStateDev *states, const QubitItemDev *items, int nitems,
                              int tbits, double2 a, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
-    const unsigned g = qubit_grid(dt, nitems);
+    const unsigned g = qubit_grid(256, nitems);
     if (dt == QOB_C64)
         hipLaunchKernelGGL((k_qubit_axpy<double2, 256>), dim3(g), dim3(256), 0, s, states, items, nitems, tbits, a);
     else
