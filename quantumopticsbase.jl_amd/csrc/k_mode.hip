@@ -158,6 +158,9 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_fib(const cplx<R> *__restrict_
 // ---------------------------------------------------------- k_mode_chunk
 // Chunks of cpc consecutive i3 blocks (cpc * n1 * d elements, contiguous).
 constexpr int CHUNK_NT = 256;  // threads per workgroup: small chunks, several workgroups per CU
+// input elements per thread loaded one chunk ahead (a ComplexF64 chunk of
+// 32 KB of planes holds 512 input elements for square factors, ComplexF32 1024)
+template <typename R> constexpr int chunk_pf() { return sizeof(R) == 8 ? 2 : 4; }
 template <typename R, int NT16>
 __global__ void __launch_bounds__(CHUNK_NT) k_mode_chunk(const cplx<R> *__restrict__ F, int d_out, int d_in,
                                                          const cplx<R> *__restrict__ in, cplx<R> *__restrict__ out,
@@ -177,16 +180,47 @@ __global__ void __launch_bounds__(CHUNK_NT) k_mode_chunk(const cplx<R> *__restri
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
+    // The first CHUNK_PF * CHUNK_NT input elements of the next chunk are
+    // loaded into registers while this chunk is multiplied and written out
+    // (the rest of a larger chunk is loaded when it is staged).
+    auto chunk_in = [&](int64_t c, int &ne) -> const T * {
+        const int64_t i30 = c * cpc;
+        const int nb = (int)(n3 - i30 < cpc ? n3 - i30 : cpc);
+        ne = nb * n1 * d_in;
+        return in + i30 * n1 * d_in;
+    };
+    constexpr int CHUNK_PF = chunk_pf<R>();
+    T pre[CHUNK_PF];
+    auto prefetch = [&](int64_t c) {
+        int ne = 0;
+        const T *src = c < nchunks ? chunk_in(c, ne) : in;
+#pragma unroll
+        for (int j = 0; j < CHUNK_PF; ++j) {
+            const int e = tid + j * CHUNK_NT;
+            pre[j] = e < ne ? ldg(src + e) : czero<T>();
+        }
+    };
+    prefetch(blockIdx.x);
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const int64_t i30 = c * cpc;
         const int nb = (int)(n3 - i30 < cpc ? n3 - i30 : cpc);
-        const int ne_in = nb * n1 * d_in, ne_out = nb * n1 * d_out;
-        const T *src = in + i30 * n1 * d_in;
-        for (int e = tid; e < ne_in; e += CHUNK_NT) {
-            const T v = src[e];
+        const int ne_out = nb * n1 * d_out;
+        int ne_in;
+        const T *src = chunk_in(c, ne_in);
+#pragma unroll
+        for (int j = 0; j < CHUNK_PF; ++j) {
+            const int e = tid + j * CHUNK_NT;
+            if (e < ne_in) {
+                ire[pad16(e)] = pre[j].x;
+                iim[pad16(e)] = pre[j].y;
+            }
+        }
+        for (int e = tid + CHUNK_PF * CHUNK_NT; e < ne_in; e += CHUNK_NT) {
+            const T v = ldg(src + e);
             ire[pad16(e)] = v.x;
             iim[pad16(e)] = v.y;
         }
+        prefetch(c + gridDim.x);
         // input planes loaded; the previous chunk's output planes are copied out
         __syncthreads();
         const int nf = nb * n1, ng = (nf + 15) >> 4;
