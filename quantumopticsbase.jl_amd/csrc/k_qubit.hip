@@ -363,9 +363,14 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
                                                QubitStateDev single, int ntiles1, QubitSplit xs) {
     constexpr int EPT = (1 << TBITS) / NTH;
     extern __shared__ unsigned char lds_raw[];
-    T *tile = (T *)lds_raw;
     const T alpha = to_t<T>(alpha_), beta = to_t<T>(beta_), gamma = to_t<T>(gamma_);
     const bool beta_zero = flags & 1, gamma_zero = flags & 2;
+    // flags & 256: two tile buffers in LDS, used by alternate items.  A
+    // buffer is rewritten two items after its last reads, and every wave has
+    // passed the barrier after the last item's store since then, so one
+    // barrier per item suffices.
+    const bool dbuf = flags & 256;
+    int par = 0;
     const int tid = threadIdx.x;
     // per-XCD ranges keep the partner tiles of a state's high-bit flips in the
     // L2 of the XCD that reads them; rounds of G keep all workgroups on
@@ -410,7 +415,9 @@ __global__ void __launch_bounds__(NTH) k_qubit(const QubitStateDev *__restrict__
             const int e = tid + k * NTH;
             xv[k] = e < tsize ? ldg(in + base + e) : czero<T>();
         }
-        __syncthreads();  // the previous item's tile reads are finished
+        T *tile = (T *)lds_raw + (par << TBITS);
+        par ^= (int)dbuf;
+        if (!dbuf) __syncthreads();  // the previous item's tile reads are finished
 #pragma unroll
         for (int k = 0; k < EPT; ++k) tile[tid + k * NTH] = xv[k];
         __syncthreads();
@@ -592,6 +599,8 @@ int g_qubit_remap = 2;
 // of workgroups walking the XCD's range, halved k - 1 times, are read with
 // non-temporal loads; 0: none
 int g_qubit_ntfar = 1;
+// 1: two tile buffers in LDS when they fit (one barrier per item instead of two)
+int g_qubit_dbuf = 1;
 
 // flags bits 4 / 8 and the item ranges of a launch with g workgroups
 static int qubit_walk_flags(int nitems, unsigned g, const QubitSplit *xs, QubitSplit &use) {
@@ -658,13 +667,17 @@ static hipError_t launch_qubit_any(qob_dtype dt, const QubitStateDev *states, co
     if (nitems == 0) return hipSuccess;
     if (tbits < 10) tbits = 10;
     if (tbits > 13) tbits = 13;
-    const size_t lds = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const size_t tile_bytes = (size_t)(1 << tbits) * (dt == QOB_C64 ? 16 : 8);
+    if (tile_bytes > 160 * 1024) return hipErrorInvalidValue;
     const int nth = qubit_nth(dt, tbits);
     const unsigned g = qubit_grid(nth, nitems);
+    // a second buffer only pays where a workgroup walks several items
+    const bool dbuf = g_qubit_dbuf && 2 * tile_bytes <= 160 * 1024 && (int64_t)g < nitems;
+    const size_t lds = dbuf ? 2 * tile_bytes : tile_bytes;
     QubitSplit ws;
     const int flags = (beta.x == 0.0 && beta.y == 0.0 ? 1 : 0) | (gamma.x == 0.0 && gamma.y == 0.0 ? 2 : 0) |
-                      qubit_walk_flags(nitems, g, xs, ws) | (g_qubit_ntfar ? 16 | ((g_qubit_ntfar - 1) << 5) : 0);
+                      qubit_walk_flags(nitems, g, xs, ws) | (g_qubit_ntfar ? 16 | ((g_qubit_ntfar - 1) << 5) : 0) |
+                      (dbuf ? 256 : 0);
 #define QOB_QB(TT, NT, TB)                                                                                    \
     do {                                                                                                      \
         auto kfn = k_qubit<TT, NT, TB>;                                                                       \
