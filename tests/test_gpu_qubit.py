@@ -304,3 +304,40 @@ def test_qubit_tile_configs_vs_oracle(split, tb, nth):
     finally:
         for k, v in saved.items():
             _lib.call("qob_tune_set", k.encode(), v)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 1e-5)])
+def test_batch_schedules_bitwise_equal(dtype, tol):
+    """Tile walk order (index / Gray code / Gray code above 64 tiles), one or
+    two LDS tile buffers, tile sizes 2^11 / 2^12 and persistent grids small
+    enough that every workgroup walks several items: the same bits, and the
+    oracle's values."""
+    names = ("qubit_order", "qubit_dbuf", "qubit_tbits", "qubit_grid")
+    saved = {k: _tune_get(k) for k in names}
+    Ls = [9, 11, 12, 13, 15, 17, 17]
+    specs, ops, psis = _tfim_states(Ls, 77)
+    al, be, ga = 0.3 - 0.2j, 0.5, -0.05j
+    r0 = [np.full(2 ** L, 0.25 + 0.5j) for L in Ls]
+    got = []
+    try:
+        for order, dbuf, tb, grid in [(0, 0, 11, 16), (1, 1, 12, 16), (2, 1, 11, 24), (1, 0, 12, 0), (1, 1, 0, 8)]:
+            for k, v in zip(names, (order, dbuf, tb, grid)):
+                _lib.call("qob_tune_set", k.encode(), v)
+            bt = Q.Batch(ops, dtype)
+            ins = [Q.DeviceArray.from_numpy(p, dtype) for p in psis]
+            outs = [Q.DeviceArray.from_numpy(r, dtype) for r in r0]
+            bt.bind(0, ins, outs)
+            bt.apply(0, al, be, ga)
+            got.append([o.to_numpy() for o in outs])
+            del bt
+    finally:
+        for k, v in saved.items():
+            _lib.call("qob_tune_set", k.encode(), v)
+    for other in got[1:]:
+        for a, b in zip(got[0], other):
+            assert np.array_equal(a, b)
+    for spec, p, r, o in zip(specs, psis, r0, got[0]):
+        want = r.copy()
+        O.mul_ket(want, _oracle_of(spec), p, al, be)
+        want = want + ga * p
+        assert O.rel_frobenius(o, want) < tol
