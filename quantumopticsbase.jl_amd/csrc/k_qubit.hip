@@ -98,28 +98,38 @@ template <bool NT, typename T> __device__ __forceinline__ T qb_ld(const T *p) {
     else return ldg(p);
 }
 
+// acc[k] += w * x[k] with w uniform, updated in place: the w.y terms first
+// (skipped for a real weight), then the w.x terms, which is cfma's order, so
+// both cases give cfma's bits (a real weight never multiplies a NaN
+// imaginary partner part).  Both branches write the accumulators where they
+// are, so the compiler needs no copies to merge them.
+template <typename T, int EPT>
+__device__ __forceinline__ void qb_acc(T wt, const T (&x)[EPT], T (&acc)[EPT]) {
+    if (wt.y != 0) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            acc[k].x = fma(-wt.y, x[k].y, acc[k].x);
+            acc[k].y = fma(wt.y, x[k].x, acc[k].y);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        acc[k].x = fma(wt.x, x[k].x, acc[k].x);
+        acc[k].y = fma(wt.x, x[k].y, acc[k].y);
+    }
+}
+
 // GL: basep points to global memory (a partner tile), else to the LDS tile
 template <typename T, int NTH, int EPT, bool GL = false, bool NT = false>
 __device__ __forceinline__ void qb_uniform(T wt, const char *basep, uint32_t tpb, T (&acc)[EPT]) {
-    auto rd = [&](int k) -> T {
+    T x[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
         const T *p = (const T *)(basep + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T))));
-        if constexpr (GL) return qb_ld<NT>(p);
-        else return *p;
-    };
-    if (wt.y == 0) {
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const T x = rd(k);
-            acc[k].x = fma(wt.x, x.x, acc[k].x);
-            acc[k].y = fma(wt.x, x.y, acc[k].y);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const T x = rd(k);
-            acc[k] = cfma(wt, x, acc[k]);
-        }
+        if constexpr (GL) x[k] = qb_ld<NT>(p);
+        else x[k] = *p;
     }
+    qb_acc<T, EPT>(wt, x, acc);
 }
 
 // acc += w * x[k ^ f] for every element k of the lane: a flip of tile bits
@@ -131,16 +141,10 @@ __device__ __forceinline__ void qb_regflip(T wt, const T (&xv)[EPT], uint32_t f,
 #pragma unroll
     for (int ff = 0; ff < EPT; ++ff) {
         if (f == (uint32_t)ff) {
-            if (wt.y == 0) {
+            T x[EPT];
 #pragma unroll
-                for (int k = 0; k < EPT; ++k) {
-                    acc[k].x = fma(wt.x, xv[k ^ ff].x, acc[k].x);
-                    acc[k].y = fma(wt.x, xv[k ^ ff].y, acc[k].y);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) acc[k] = cfma(wt, xv[k ^ ff], acc[k]);
-            }
+            for (int k = 0; k < EPT; ++k) x[k] = xv[k ^ ff];
+            qb_acc<T, EPT>(wt, x, acc);
         }
     }
 }
@@ -219,18 +223,7 @@ __device__ __forceinline__ void qb_run(const int32_t *__restrict__ pg, const T *
                     dst[k] = ldg((const T *)(gp + (tpb ^ (uint32_t)(k * NTH * (int)sizeof(T)))));
             }
         };
-        auto accum = [&](T wt, const T (&x)[EPT]) {
-            if (wt.y == 0) {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) {
-                    acc[k].x = fma(wt.x, x[k].x, acc[k].x);
-                    acc[k].y = fma(wt.x, x[k].y, acc[k].y);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < EPT; ++k) acc[k] = cfma(wt, x[k], acc[k]);
-            }
-        };
+        auto accum = [&](T wt, const T (&x)[EPT]) { qb_acc<T, EPT>(wt, x, acc); };
         // Constant groups are in ascending F: groups [0, nin) flip bits inside
         // the tile (header word 2), the rest read partner tiles.  Partner
         // loads run one group ahead through two buffers, the first group's
