@@ -3376,17 +3376,21 @@ qob_status qob_batch_create(qob_ctx ctx, int n, const qob_op *ops, qob_dtype dt,
     B->dt = dt;
     B->n = n;
     B->tbits = qubit_tbits(dt);
-    if (!g_qubit_tbits && B->tbits > 11) {
-        // tiles larger than 2^11 only when most of the elements are in states
-        // at least that large (a smaller state leaves lanes of its tile idle)
-        int64_t big = 0, all = 0;
-        for (int j = 0; j < n; ++j) {
+    if (!g_qubit_tbits) {
+        // the largest tile (down to 2^10) that states holding most of the
+        // elements fill: a state smaller than the tile leaves lanes idle
+        for (int j = 0; j < n; ++j)
             if (!ops[j]) return fail(QOB_EARG, "null operator %d", j);
-            const int64_t d = ops[j]->node->n;
-            all += d;
-            if (d >= ((int64_t)1 << B->tbits)) big += d;
+        while (B->tbits > 10) {
+            int64_t big = 0, all = 0;
+            for (int j = 0; j < n; ++j) {
+                const int64_t d = ops[j]->node->n;
+                all += d;
+                if (d >= ((int64_t)1 << B->tbits)) big += d;
+            }
+            if (2 * big >= all) break;
+            --B->tbits;
         }
-        if (2 * big < all) B->tbits = 11;
     }
     std::vector<int32_t> prog;
     std::vector<cd> vals;
