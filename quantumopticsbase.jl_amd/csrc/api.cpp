@@ -1920,8 +1920,14 @@ static qob_status apply_lt_chain(const Apply &ap, OpNode &op, const void *in, vo
                     src = t1;
                 }
                 const int64_t nrb = ap.lead / 16;
-                hipError_t e = launch_lt_fused(ap.dt, FM, src, out, nrb * (Din / P), d2(a), d2(beta), ap.s,
-                                               ap.lead, nrb);
+                // two factors filling the tile: chained in registers
+                hipError_t e = launch_lt_reg_tile(ap.dt, FM, src, out, nrb * (Din / P), nrb, ap.lead, d2(a),
+                                                  d2(beta), ap.s);
+                if (e == hipSuccess) return QOB_OK;
+                if (e != hipErrorNotSupported)
+                    return fail(QOB_EHIP, "LazyTensor register-chain tile launch: %s", hipGetErrorString(e));
+                (void)hipGetLastError();
+                e = launch_lt_fused(ap.dt, FM, src, out, nrb * (Din / P), d2(a), d2(beta), ap.s, ap.lead, nrb);
                 if (e != hipSuccess) return fail(QOB_EHIP, "fused LazyTensor tile launch: %s", hipGetErrorString(e));
                 return QOB_OK;
             }

@@ -217,6 +217,139 @@ __global__ void __launch_bounds__(REG_NT) k_lt_reg(const cplx<R> *__restrict__ A
     }
 }
 
+// Right application on tiles of 16 rows (R = rho * LT on a column-major
+// rho of leading dimension ld): tile c holds rows 16 (c % nrb) + 0..15 of the
+// D2 * D3 consecutive columns starting at (c / nrb) * D2 * D3, viewed as
+// [r (16), i2 (D2), i3 (D3)] with element (r, i2, i3) at
+// base + r + ld (i2 + D2 i3).  The factors act on i2 and i3 (the rows are a
+// batch index), so this is the column kernel above without a site-1 factor:
+// a block (16 rows x D2 columns, fixed i3) is loaded straight into mode 2's
+// B-operand layout (k = i2 along lane / 16, the 16 rows along lane % 16:
+// four 256-byte runs per load instruction in ComplexF64), mode 2 runs on it
+// in registers and mode 3 goes through the LDS planes as above.  Mode 2 is
+// short (one MFMA k-chain per block), so a wave's blocks of the next tile
+// are all loaded while this tile's mode 3 runs; the site-3 factor sits in
+// LDS beside the planes to leave registers for them.
+template <typename R, int D2, int D3>
+__global__ void __launch_bounds__(REG_NT) k_lt_reg_tile(const cplx<R> *__restrict__ A2, const cplx<R> *__restrict__ A3,
+                                                         const cplx<R> *__restrict__ in, cplx<R> *__restrict__ out,
+                                                         int64_t ntiles, int64_t nrb, int64_t ld, double2 alpha_,
+                                                         double2 beta_, int beta_zero) {
+    using T = cplx<R>;
+    using V = typename mf<R>::v4;
+    constexpr int BLK = 16 * D2;     // elements per block
+    constexpr int NT2 = D2 / 16, KS2 = D2 / 4;
+    constexpr int NT3 = D3 / 16, KS3 = D3 / 4;
+    constexpr int NB = D3;           // blocks per tile
+    constexpr int D = BLK * NB;      // elements per tile
+    constexpr int WPB = REG_NT / 64;
+    constexpr int BPW = NB / WPB;    // blocks per wave (NB is 16 or 32)
+    constexpr int LDA2 = D2 + 1, LDA3 = D3 + 1;
+    static_assert(NB % WPB == 0, "blocks per tile must be a multiple of the waves");
+    extern __shared__ unsigned char lds_raw[];
+    T *A2s = (T *)lds_raw;
+    T *A3s = A2s + D2 * LDA2;
+    constexpr int Dp = D + (D >> 4) + 1;
+    R *yre = (R *)(A3s + D3 * LDA3);
+    R *yim = yre + Dp;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const T alpha = mk((R)alpha_.x, (R)alpha_.y), beta = mk((R)beta_.x, (R)beta_.y);
+
+    for (int e = tid; e < D2 * D2; e += REG_NT) A2s[(e % D2) + (e / D2) * LDA2] = A2[e];
+    for (int e = tid; e < D3 * D3; e += REG_NT) A3s[(e % D3) + (e / D3) * LDA3] = A3[e];
+    __syncthreads();  // A2 and A3 staged
+
+    auto tile_base = [&](int64_t c) -> int64_t { return 16 * (c % nrb) + ld * (int64_t)D2 * NB * (c / nrb); };
+    // the wave's blocks b = wave + WPB q: mode-2 B-fragments, k-step ks
+    // holding (r = lane % 16, i2 = 4 ks + lane / 16)
+    T x[BPW][KS2];
+    auto load_tile = [&](int64_t base) {
+#pragma unroll
+        for (int q = 0; q < BPW; ++q)
+#pragma unroll
+            for (int ks = 0; ks < KS2; ++ks)
+                x[q][ks] = in[base + lr + ld * (int64_t)(4 * ks + lk + D2 * (wave + WPB * q))];
+    };
+    // mode 2 of block q: tile u of D2 = A2 M (rows i2' = 16 u + mf<R>::row(lane, r), columns r = lane % 16)
+    auto mode2 = [&](const T *F2, int q, V (&d2r)[NT2], V (&d2i)[NT2]) {
+#pragma unroll
+        for (int u = 0; u < NT2; ++u) {
+            d2r[u] = V{0, 0, 0, 0};
+            d2i[u] = V{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS2; ++ks)
+#pragma unroll
+            for (int u = 0; u < NT2; ++u) cmma4<R>(F2[(16 * u + lr) + (4 * ks + lk) * LDA2], x[q][ks], d2r[u], d2i[u]);
+    };
+    auto put = [&](int blk, const V (&d2r)[NT2], const V (&d2i)[NT2]) {
+#pragma unroll
+        for (int u = 0; u < NT2; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int e = blk * BLK + lr + 16 * (16 * u + mf<R>::row(lane, r));
+                yre[pad16r(e)] = d2r[u][r];
+                yim[pad16r(e)] = d2i[u][r];
+            }
+    };
+    int64_t c = blockIdx.x;
+    if (c < ntiles) load_tile(tile_base(c));
+    for (; c < ntiles; c += gridDim.x) {
+        const int64_t base = tile_base(c);
+        const int64_t nc = c + gridDim.x;
+        // the factors stay in LDS: an offset the compiler cannot see through
+        // keeps their reads inside the loop instead of hoisting them into
+        // registers for the whole launch
+        int fo = 0;
+        asm volatile("" : "+s"(fo));
+        const T *F2 = A2s + fo, *F3 = A3s + fo;
+        {
+            // the first block is formed while other waves may still read the
+            // previous tile's planes, and written after the barrier
+            V d2r[NT2], d2i[NT2];
+            mode2(F2, 0, d2r, d2i);
+            __syncthreads();  // every wave is done reading the previous tile's planes
+            put(wave, d2r, d2i);
+#pragma unroll
+            for (int q = 1; q < BPW; ++q) {
+                mode2(F2, q, d2r, d2i);
+                put(wave + WPB * q, d2r, d2i);
+            }
+        }
+        __syncthreads();  // the tile's planes are complete
+        if (nc < ntiles) load_tile(tile_base(nc));
+        // mode 3 over groups of 16 plane elements e = r + 16 i2 (r = lane % 16)
+        for (int g = wave; g < BLK / 16; g += WPB) {
+            const int e0 = 16 * g;
+            V zr[NT3], zi[NT3];
+#pragma unroll
+            for (int v = 0; v < NT3; ++v) {
+                zr[v] = V{0, 0, 0, 0};
+                zi[v] = V{0, 0, 0, 0};
+            }
+            // two k-steps at a time: the next tile's loads are in flight in
+            // registers meanwhile
+#pragma unroll 2
+            for (int ks = 0; ks < KS3; ++ks) {
+                const int idx = pad16r(e0 + lr + BLK * (4 * ks + lk));
+                const T yv = mk(yre[idx], yim[idx]);
+#pragma unroll
+                for (int v = 0; v < NT3; ++v) cmma4<R>(F3[(16 * v + lr) + (4 * ks + lk) * LDA3], yv, zr[v], zi[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < NT3; ++v)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    T *o = out + base + lr + ld * (int64_t)(g + D2 * (16 * v + mf<R>::row(lane, r)));
+                    T val = cmul(alpha, mk((R)zr[v][r], (R)zi[v][r]));
+                    if (!beta_zero) val = cfma(beta, *o, val);
+                    st_stream(o, val);
+                }
+        }
+    }
+}
+
 }  // namespace
 
 int g_fused_reg = 1;  // 1: this kernel where the factors' sites and sizes fit it
@@ -282,6 +415,54 @@ hipError_t launch_lt_reg(qob_dtype dt, const FusedModes &M, const void *in, void
         if (d2 == 32 && d3 == 16) QOB_LR(float, 32, 16);
     }
 #undef QOB_LR
+    return hipErrorNotSupported;
+}
+
+// Right application in 16-row tiles (see k_lt_reg_tile): the tile's two
+// factors sit on its sites 2 and 3 (FusedModes n1 = 16 and 16 d2, the
+// tile exactly 16 x d2 x d3 elements), d2 and d3 in {16, 32}, not both 32.
+// Returns hipErrorNotSupported otherwise (the caller runs the fused column
+// kernel on the tiles).
+hipError_t launch_lt_reg_tile(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ntiles,
+                              int64_t nrb, int64_t ld, double2 alpha, double2 beta, hipStream_t s) {
+    if (!g_fused_reg || M.nmodes != 2 || M.n1[0] != 16) return hipErrorNotSupported;
+    const int d2 = M.d[0], d3 = M.d[1];
+    if ((d2 != 16 && d2 != 32) || (d3 != 16 && d3 != 32) || (d2 == 32 && d3 == 32)) return hipErrorNotSupported;
+    if (M.n1[1] != 16 * d2 || M.D != 16 * d2 * d3) return hipErrorNotSupported;
+    if (nrb < 1 || ld < 16 * nrb || ntiles < 0 || ntiles % nrb) return hipErrorInvalidValue;
+    const size_t lds = lt_reg_lds(dt, d2, M.D) + (size_t)d3 * (d3 + 1) * (dt == QOB_C64 ? 16 : 8);  // + A3
+    if (lds > 160 * 1024) return hipErrorNotSupported;
+    if (ntiles == 0) return hipSuccess;
+    const int bz = (beta.x == 0.0 && beta.y == 0.0);
+    const int ncu = device_cu_count();
+#define QOB_LT(RR, DD2, DD3)                                                                                    \
+    do {                                                                                                        \
+        auto k = k_lt_reg_tile<RR, DD2, DD3>;                                                                   \
+        static std::atomic<int> nres{0};                                                                        \
+        if (!nres.load()) {                                                                                     \
+            hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                               160 * 1024);                                                     \
+            if (e != hipSuccess) return e;                                                                      \
+            int nb = 0;                                                                                         \
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, REG_NT, lds);                              \
+            if (e != hipSuccess) return e;                                                                      \
+            nres.store(nb > 0 ? nb : 1);                                                                        \
+        }                                                                                                       \
+        const unsigned g = (unsigned)std::min<int64_t>(ntiles, (int64_t)ncu * nres.load());                     \
+        hipLaunchKernelGGL(k, dim3(g), dim3(REG_NT), lds, s, (const cplx<RR> *)M.A[0], (const cplx<RR> *)M.A[1], \
+                           (const cplx<RR> *)in, (cplx<RR> *)out, ntiles, nrb, ld, alpha, beta, bz);            \
+        return hipGetLastError();                                                                               \
+    } while (0)
+    if (dt == QOB_C64) {
+        if (d2 == 16 && d3 == 16) QOB_LT(double, 16, 16);
+        if (d2 == 16 && d3 == 32) QOB_LT(double, 16, 32);
+        if (d2 == 32 && d3 == 16) QOB_LT(double, 32, 16);
+    } else {
+        if (d2 == 16 && d3 == 16) QOB_LT(float, 16, 16);
+        if (d2 == 16 && d3 == 32) QOB_LT(float, 16, 32);
+        if (d2 == 32 && d3 == 16) QOB_LT(float, 32, 16);
+    }
+#undef QOB_LT
     return hipErrorNotSupported;
 }
 

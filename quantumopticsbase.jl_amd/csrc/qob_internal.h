@@ -326,6 +326,8 @@ hipError_t launch_lt_fused(qob_dtype dt, const FusedModes &M, const void *in, vo
 // registers (k_ltreg.hip).  hipErrorNotSupported when the shape does not fit.
 hipError_t launch_lt_reg(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ncols,
                          double2 alpha, double2 beta, hipStream_t s);
+hipError_t launch_lt_reg_tile(qob_dtype dt, const FusedModes &M, const void *in, void *out, int64_t ntiles,
+                              int64_t nrb, int64_t ld, double2 alpha, double2 beta, hipStream_t s);
 
 // Fused reductions over a square gather plan (FP64 partials, one per block):
 //   mode 0: sum_J conj(u[J]) (H v)[J]          (expect / bilinear on a Ket)

@@ -874,13 +874,17 @@ def test_sparse_rho_degenerate_components(kind, dtype, tol):
 
 
 @pytest.mark.parametrize("shape,idx", [((16, 16), (1, 2)), ((16, 16, 16, 16), (1, 2, 3, 4)),
-                                       ((16, 16, 16, 16), (2, 4)), ((16, 32, 16), (1, 2, 3))])
+                                       ((16, 16, 16, 16), (2, 4)), ((16, 32, 16), (1, 2, 3)),
+                                       ((32, 16, 16), (1, 2, 3))])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])
 def test_dense_lazytensor_right_tiles(shape, idx, dtype, tol):
     """rho * LazyTensor of dense factors with 64 rows: the leading sites run
     fused over 16-row tiles, the others through the chain first (two of them
-    in the 4-site cases, one an identity site inside the tile in the third);
-    against the oracle and against the chain alone (tiles off)."""
+    in the 4-site cases, one an identity site inside the tile in the third).
+    Tiles of two factors (16 x 16, 16 x 32, 32 x 16) run the register-chained
+    tile kernel, the rest the fused column kernel on tiles; against the
+    oracle, against the chain alone (tiles off) and against the fused column
+    kernel on the tiles (register chain off)."""
     import ctypes as C
     from qobhip import _lib
     g = np.random.default_rng(sum(shape) + len(idx))
@@ -888,12 +892,14 @@ def test_dense_lazytensor_right_tiles(shape, idx, dtype, tol):
     D = int(np.prod(shape))
     rows = 64
     B = (g.standard_normal((rows, D)) + 1j * g.standard_normal((rows, D))) / np.sqrt(D)
-    v = C.c_int64()
+    v, vr = C.c_int64(), C.c_int64()
     _lib.call("qob_tune_get", b"fused_tiles", C.byref(v))
+    _lib.call("qob_tune_get", b"fused_reg", C.byref(vr))
     got = {}
     try:
-        for tiles in (1, 0):
+        for tiles, reg in ((1, 1), (0, 1), (1, 0)):
             _lib.call("qob_tune_set", b"fused_tiles", tiles)
+            _lib.call("qob_tune_set", b"fused_reg", reg)
             q = to_q(lt)
             for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
                 R0 = np.full((rows, D), np.nan, complex) if beta == 0 else g.standard_normal((rows, D)) + 0j
@@ -903,10 +909,12 @@ def test_dense_lazytensor_right_tiles(shape, idx, dtype, tol):
                 Q.mul_(R, Q.Operator(gb(rows), q.basis_l, B, dtype), q, alpha, beta)
                 assert rel(R.to_numpy(), want) < tol
                 if beta == 0:
-                    got[tiles] = R.to_numpy()
+                    got[(tiles, reg)] = R.to_numpy()
     finally:
         _lib.call("qob_tune_set", b"fused_tiles", v.value)
-    assert rel(got[1], got[0]) < tol
+        _lib.call("qob_tune_set", b"fused_reg", vr.value)
+    assert rel(got[(1, 1)], got[(0, 1)]) < tol
+    assert rel(got[(1, 1)], got[(1, 0)]) < tol
 
 
 def _q_lazyproduct(lp):
