@@ -341,3 +341,36 @@ def test_batch_schedules_bitwise_equal(dtype, tol):
         O.mul_ket(want, _oracle_of(spec), p, al, be)
         want = want + ga * p
         assert O.rel_frobenius(o, want) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 1e-5)])
+def test_qubit_columns_persistent_walks(dtype, tol):
+    """A qubit sum on the 16 columns of a density-matrix slice with grids
+    small enough that each workgroup walks several (column, tile) items, with
+    one and two LDS tile buffers and with the XCD ranges: against the oracle,
+    and the same bits every time."""
+    names = ("qubit_grid", "qubit_dbuf", "qubit_remap")
+    saved = {k: _tune_get(k) for k in names}
+    g = np.random.default_rng(913)
+    L, ncol = 13, 16
+    Hq, Ho, B = build_pair(L, random_pauli_sum(g, L, 18, maxspan=13))
+    D = 2 ** L
+    Bm, R0 = cg(g, D, ncol), cg(g, D, ncol)
+    al, be = 0.6 + 0.1j, -0.3j
+    want = R0.copy()
+    O.mul_op_left(want, Ho, Bm, al, be)
+    cb = Q.GenericBasis(ncol)
+    got = []
+    try:
+        for grid, dbuf, remap in [(0, 1, 2), (8, 1, 2), (8, 0, 2), (3, 1, 2), (5, 1, 1)]:
+            for k, v in zip(names, (grid, dbuf, remap)):
+                _lib.call("qob_tune_set", k.encode(), v)
+            R = Q.Operator(B, cb, R0, dtype)
+            Q.mul_(R, Hq, Q.Operator(B, cb, Bm, dtype), al, be)
+            got.append(R.to_numpy())
+    finally:
+        for k, v in saved.items():
+            _lib.call("qob_tune_set", k.encode(), v)
+    assert O.rel_frobenius(got[0], want) < tol
+    for other in got[1:]:
+        assert np.array_equal(got[0], other)
