@@ -1870,6 +1870,13 @@ static qob_status apply_lt_chain(const Apply &ap, OpNode &op, const void *in, vo
             if (e == hipSuccess) return QOB_OK;
             if (e != hipErrorNotSupported) return fail(QOB_EHIP, "LazyTensor register-chain launch: %s", hipGetErrorString(e));
             (void)hipGetLastError();
+            // no factor on a first site of dimension 16, factors on sites 2
+            // and 3: a column is a tile of 16 "rows" with leading dimension 16
+            e = launch_lt_reg_tile(ap.dt, FM, in, out, ap.trail, 1, 16, d2(a), d2(beta), ap.s);
+            if (e == hipSuccess) return QOB_OK;
+            if (e != hipErrorNotSupported)
+                return fail(QOB_EHIP, "LazyTensor register-chain tile launch: %s", hipGetErrorString(e));
+            (void)hipGetLastError();
         }
         if (ok && fused_supported(ap.dt, FM)) {
             hipError_t e = launch_lt_fused(ap.dt, FM, in, out, ap.trail, d2(a), d2(beta), ap.s);

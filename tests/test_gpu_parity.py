@@ -382,6 +382,9 @@ FUSED_CASES = [
     ((2, 16, 16), (2, 3)),      # identity fastest site: n1 = 2
     ((16, 32, 16), (1, 2, 3)),
     ((64, 16), (1, 2)),
+    ((16, 16, 32), (2, 3)),     # no site-1 factor: columns as tiles of 16 rows
+    ((16, 32, 16), (2, 3)),
+    ((16, 16, 16), (2, 3)),
 ]
 
 
