@@ -33,7 +33,7 @@ namespace qob {
 
 extern int g_sell_variant, g_sell_cj, g_sell_jchunk, g_csc_rb, g_csc_cc, g_csc_remap, g_nt_store, g_copy_variant;
 int g_fused_lt = 1;  // fused MFMA column kernel for dense LazyTensors
-extern int g_qubit_nth, g_qubit_grid, g_qubit_remap, g_qubit_ntfar, g_qubit_dbuf;
+extern int g_qubit_nth, g_qubit_grid, g_qubit_remap, g_qubit_ntfar, g_qubit_dbuf, g_qubit_pair;
 // order of a run's tiles in a batch (batches created after it is set):
 // 0 index order; 1 binary-reflected Gray code of the index, so that tiles
 // walked in turn differ in one bit and the partner runs of the high bits
@@ -2967,7 +2967,7 @@ static int *tune_slot(const char *name) {
                {"qubit_grid", &g_qubit_grid},     {"qubit_remap", &g_qubit_remap},
                {"qubit_split", &g_qubit_split},   {"mode_kernel", &g_mode_kernel}, {"fused_reg", &g_fused_reg},
                {"qubit_ntfar", &g_qubit_ntfar}, {"qubit_order", &g_qubit_order},
-               {"qubit_dbuf", &g_qubit_dbuf},
+               {"qubit_dbuf", &g_qubit_dbuf}, {"qubit_pair", &g_qubit_pair},
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
@@ -3011,7 +3011,8 @@ qob_status qob_tune_set(const char *name, int64_t value) {
          !strcmp(name, "sell_cline") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
          !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles") ||
-         !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg") || !strcmp(name, "qubit_dbuf")) &&
+         !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg") || !strcmp(name, "qubit_dbuf") ||
+         !strcmp(name, "qubit_pair")) &&
         value > 1)
         return fail(QOB_EARG, "%s must be 0 or 1", name);
     if (!strcmp(name, "qubit_nth") && value != 0 && value != 256 && value != 512 && value != 1024)

@@ -18,7 +18,8 @@ of libqob_hip.so (qob_tune_set / qob_tune_get):
                     tiles outside the run in flight), qubit_order (tile walk
                     order of batches created after it is set: 0 index, 1 Gray
                     code, 2 Gray code above 64 tiles), qubit_dbuf (two LDS tile
-                    buffers in persistent walks), qubit_split
+                    buffers in persistent walks), qubit_pair (ComplexF32
+                    lanes on pairs of adjacent elements), qubit_split
   sparse products   sell_variant, sell_cj, sell_jchunk, sell_unr, csc_rows,
                     csc_rb, csc_cc, csc_remap, csc_order (column sweep order),
                     csc_sweep (row stripes / column-major / by reuse
@@ -55,7 +56,7 @@ from ._lib import call
 SPACES: Dict[str, Dict[str, List[int]]] = {
     "fused": {"fused_gpw": [0, 1, 2, 4], "fused_resident": [1, 0]},
     "qubit": {"qubit_tbits": [0, 10, 11, 12, 13], "qubit_nth": [0, 256, 512, 1024], "qubit_grid": [0, 256, 512, 1024],
-              "qubit_remap": [2, 1, 0], "qubit_ntfar": [1, 0], "qubit_order": [1, 0, 2], "qubit_dbuf": [1, 0], "qubit_split": [0, 1]},
+              "qubit_remap": [2, 1, 0], "qubit_ntfar": [1, 0], "qubit_order": [1, 0, 2], "qubit_dbuf": [1, 0], "qubit_pair": [1, 0], "qubit_split": [0, 1]},
     "sparse_left": {"sell_variant": [0, 1], "sell_cj": [1, 2, 4, 8], "sell_jchunk": [4, 8, 16, 32],
                     "sell_unr": [2, 4, 8], "sell_cjchunk": [4, 2, 8], "sell_cremap": [1, 0]},
     "sparse_right": {"csc_rows": [16, 32, 64], "csc_rb": [64, 128, 256], "csc_cc": [1, 2, 4, 8], "csc_remap": [0, 1],

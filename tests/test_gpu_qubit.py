@@ -374,3 +374,69 @@ def test_qubit_columns_persistent_walks(dtype, tol):
     assert O.rel_frobenius(got[0], want) < tol
     for other in got[1:]:
         assert np.array_equal(got[0], other)
+
+
+@pytest.mark.parametrize("L,nterms", [(1, 3), (3, 6), (9, 14), (12, 18), (15, 16)])
+def test_complexf32_pair_lanes_bitwise(L, nterms):
+    """ComplexF32 with lanes holding pairs of adjacent elements (16-byte
+    accesses) against single elements: the same bits on a Ket, a Bra and
+    density-matrix columns (random Pauli sums with diagonal windows, flips of
+    bit 0, register bits and partner tiles), and against the oracle."""
+    dtype, tol = np.complex64, 1e-5
+    g = np.random.default_rng(500 + L)
+    Hq, Ho, B = build_pair(L, random_pauli_sum(g, L, nterms, maxspan=min(L, 12)))
+    D = 2 ** L
+    al, be, ncol = 0.7 - 0.2j, 0.3 + 0.4j, 3
+    psi, r0 = cg(g, D), cg(g, D)
+    Bm, R0 = cg(g, D, ncol), cg(g, D, ncol)
+    cb = Q.GenericBasis(ncol)
+    saved = _tune_get("qubit_pair")
+    got = {}
+    try:
+        for pair in (0, 1):
+            _lib.call("qob_tune_set", b"qubit_pair", pair)
+            r = Q.Ket(B, r0, dtype)
+            Q.mul_(r, Hq, Q.Ket(B, psi, dtype), al, be)
+            rb = Q.Bra(B, r0, dtype)
+            Q.mul_(rb, Q.Bra(B, psi, dtype), Hq, al, be)
+            R = Q.Operator(B, cb, R0, dtype)
+            Q.mul_(R, Hq, Q.Operator(B, cb, Bm, dtype), al, be)
+            got[pair] = (r.to_numpy(), rb.to_numpy(), R.to_numpy())
+    finally:
+        _lib.call("qob_tune_set", b"qubit_pair", saved)
+    for a, b in zip(got[0], got[1]):
+        assert np.array_equal(a, b)
+    want = r0.copy()
+    O.mul_ket(want, Ho, psi, al, be)
+    assert O.rel_frobenius(got[1][0], want) < tol
+    want = R0.copy()
+    O.mul_op_left(want, Ho, Bm, al, be)
+    assert O.rel_frobenius(got[1][2], want) < tol
+
+
+def test_complexf32_pair_lanes_batch_bitwise():
+    """The 256-chain-like batch path (partner tiles, per-XCD walks, Gray
+    order, fused gamma) with pair lanes against single elements."""
+    Ls = [9, 11, 12, 13, 15, 17, 17]
+    specs, ops, psis = _tfim_states(Ls, 78)
+    al, be, ga = 0.3 - 0.2j, 0.5, -0.05j
+    r0 = [np.full(2 ** L, 0.25 + 0.5j) for L in Ls]
+    saved = {k: _tune_get(k) for k in ("qubit_pair", "qubit_grid")}
+    got = []
+    try:
+        for pair, grid in [(0, 0), (1, 0), (1, 16)]:
+            _lib.call("qob_tune_set", b"qubit_pair", pair)
+            _lib.call("qob_tune_set", b"qubit_grid", grid)
+            bt = Q.Batch(ops, np.complex64)
+            ins = [Q.DeviceArray.from_numpy(p, np.complex64) for p in psis]
+            outs = [Q.DeviceArray.from_numpy(r, np.complex64) for r in r0]
+            bt.bind(0, ins, outs)
+            bt.apply(0, al, be, ga)
+            got.append([o.to_numpy() for o in outs])
+            del bt
+    finally:
+        for k, v in saved.items():
+            _lib.call("qob_tune_set", k.encode(), v)
+    for other in got[1:]:
+        for a, b in zip(got[0], other):
+            assert np.array_equal(a, b)
