@@ -268,7 +268,7 @@ qob_status qob_graph_destroy(qob_graph g);
  * "csc_rows", "csc_order", "qubit_tbits", "qubit_nth", "qubit_grid",
  * "qubit_remap", "qubit_ntfar", "qubit_order", "qubit_dbuf", "qubit_pair", "qubit_split", "qubit_enable", "mode_kernel", "fused_reg", "fused_gpw",
  * "fused_lt", "fused_tiles", "fused_resident", "nt_store", "copy_variant",
- * "sell_align", "sell_closed", "sell_cline", "sell_cjchunk", "sell_cremap", "csc_groups",
+ * "sell_align", "sell_closed", "sell_cline", "sell_cpair", "sell_cjchunk", "sell_cremap", "csc_groups",
  * "csc_gkernel", "csc_gremap", "csc_waves", "csc_sweep", "ls_materialize".
  * The project's
  * autotuner (qobhip/tune.py) sweeps them and persists a table; unknown names

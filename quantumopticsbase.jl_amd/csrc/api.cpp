@@ -273,8 +273,14 @@ int g_sell_align = 1;
 // run of rows in a block is whole lines and no two blocks write into one
 // line.  The components then grow; they are packed into blocks of 320 rows
 // (5 slices), used when that fills the blocks at least 90%.
+//
+// Row-pair form (ComplexF32, sell_cpair = 1): rows 2i and 2i + 1 are joined,
+// so every block holds whole pairs at even lanes and a thread can move two
+// rows with one 16-byte access per column.  Blocks of 256 rows, else 320,
+// whichever is first filled at least 90%; else the plain 256-row form.
 int g_sell_closed = 1;
 int g_sell_cline = 1;
+int g_sell_cpair = 1;
 
 // lanes (block-major, BLK per block, -1 idle) of the closed layout with rows
 // joined in aligned groups of grp; false if a component exceeds BLK rows
@@ -331,11 +337,19 @@ static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
     const int64_t m = X.nr;
     if (m != X.nc || m < 2 || m >= (1ll << 31)) return false;
     std::vector<int32_t> lanes;
-    int spb = 4;
-    if (g_sell_cline && dt == QOB_C64 && closed_lanes(X, 8, 320, lanes) && (double)m >= 0.9 * (double)lanes.size())
-        spb = 5;
-    else if (!closed_lanes(X, 1, 256, lanes))
-        return false;
+    int spb = 0;
+    auto filled = [&](int grp, int blk) {
+        return closed_lanes(X, grp, blk, lanes) && (double)m >= 0.9 * (double)lanes.size();
+    };
+    if (g_sell_cline && dt == QOB_C64 && filled(8, 320)) spb = 5;
+    if (!spb && g_sell_cpair && dt == QOB_C32 && m % 2 == 0) {
+        if (filled(2, 256)) spb = 4;
+        else if (filled(2, 320)) spb = 5;
+    }
+    if (!spb) {
+        if (!closed_lanes(X, 1, 256, lanes)) return false;
+        spb = 4;
+    }
     const int BLK = 64 * spb;
     const int nblocks = (int)(lanes.size() / BLK), nslices = nblocks * spb;
     std::vector<int32_t> lane_of((size_t)m, -1), off(nslices + 1, 0);
@@ -378,6 +392,9 @@ static bool build_sell_closed(const RowLists &X, qob_dtype dt, SellTab &S) {
     S.dev.val = S.val.p;
     S.dev.rows = (const int32_t *)S.rows.p;
     S.dev.spb = spb;
+    S.dev.pairs = g_sell_cpair && dt == QOB_C32 && m % 2 == 0;
+    for (size_t e = 0; e < lanes.size() && S.dev.pairs; e += 2)
+        S.dev.pairs = lanes[e] < 0 ? lanes[e + 1] < 0 : lanes[e] % 2 == 0 && lanes[e + 1] == lanes[e] + 1;
     return true;
 }
 
@@ -2971,7 +2988,7 @@ static int *tune_slot(const char *name) {
                {"fused_gpw", &g_fused_gpw},       {"csc_rows", &g_csc_rows},
                {"sell_unr", &g_sell_unr},         {"csc_order", &g_csc_order},
                {"sell_align", &g_sell_align},     {"csc_waves", &g_csc_waves},
-               {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed}, {"sell_cline", &g_sell_cline},
+               {"csc_sweep", &g_csc_sweep},       {"sell_closed", &g_sell_closed}, {"sell_cline", &g_sell_cline}, {"sell_cpair", &g_sell_cpair},
                {"sell_cjchunk", &g_sell_cjchunk}, {"csc_groups", &g_csc_groups},
                {"csc_gkernel", &g_csc_gkernel},   {"csc_gremap", &g_csc_gremap},
                {"sell_cremap", &g_sell_cremap},   {"ls_materialize", &g_ls_materialize},
@@ -3008,7 +3025,7 @@ qob_status qob_tune_set(const char *name, int64_t value) {
     if (!strcmp(name, "qubit_ntfar") && value > 8) return fail(QOB_EARG, "qubit_ntfar must be in 0..8");
     if (!strcmp(name, "qubit_order") && value > 2) return fail(QOB_EARG, "qubit_order must be 0, 1 or 2");
     if ((!strcmp(name, "csc_order") || !strcmp(name, "sell_align") || !strcmp(name, "sell_closed") ||
-         !strcmp(name, "sell_cline") ||
+         !strcmp(name, "sell_cline") || !strcmp(name, "sell_cpair") ||
          !strcmp(name, "csc_groups") || !strcmp(name, "csc_gkernel") || !strcmp(name, "csc_gremap") ||
          !strcmp(name, "sell_cremap") || !strcmp(name, "ls_materialize") || !strcmp(name, "fused_tiles") ||
          !strcmp(name, "fused_resident") || !strcmp(name, "fused_reg") || !strcmp(name, "qubit_dbuf") ||

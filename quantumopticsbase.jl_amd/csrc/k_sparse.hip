@@ -343,6 +343,81 @@ __global__ void __launch_bounds__(64) k_csc_right_groups_x2(int64_t rows, const 
     }
 }
 
+// ---- left, closed blocks of row pairs (ComplexF32): lanes 2t and 2t + 1 of
+// a block hold rows 2i and 2i + 1 (or are both idle), and thread t owns both,
+// so each column of B and R moves through one 16-byte load and one 16-byte
+// store per thread.  Slices, staging in LDS and the order in which a row's
+// entries are summed are those of k_sell_left_closed.
+template <int MAXW, int UNR, int SPB>
+__global__ void __launch_bounds__(32 * SPB) k_sell_left_closed_x2(int nslices, const int32_t *__restrict__ slice_off,
+                                                                  const int32_t *__restrict__ rows,
+                                                                  const int32_t *__restrict__ col,
+                                                                  const float2 *__restrict__ val,
+                                                                  const float2 *__restrict__ B, int64_t ldb,
+                                                                  int64_t cols, float2 *__restrict__ R, int64_t ldr,
+                                                                  double2 alpha_, double2 beta_, int beta_zero,
+                                                                  uint32_t nsg, int jchunk, int remap) {
+    constexpr int BLK = 64 * SPB;
+    __shared__ float2 sb[UNR][BLK];
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = remap ? xcd_remap(blockIdx.x, G) : blockIdx.x;
+    const uint32_t sg = lb % nsg, cg = lb / nsg;
+    const int t = threadIdx.x;
+    if ((int64_t)sg * SPB >= nslices) return;  // workgroup-uniform, before any barrier
+    const int slice = (int)(sg * SPB + (t >> 5));
+    const int r = 2 * (t & 31);  // first of the thread's two lanes in its slice
+    const int32_t row = rows[(int64_t)sg * BLK + 2 * t];  // even, or -1 for an idle pair
+    const float2 alpha = to_t<float2>(alpha_), beta = to_t<float2>(beta_);
+    const int32_t o0 = slice_off[slice];
+    const int w = (slice_off[slice + 1] - o0) >> 6;
+    int2 c[MAXW];
+    f4v v[MAXW];
+#pragma unroll
+    for (int p = 0; p < MAXW; ++p) {
+        c[p] = (p < w) ? *(const int2 *)(col + o0 + p * 64 + r) : make_int2(-1, -1);
+        v[p] = (p < w) ? *(const f4v *)(val + o0 + p * 64 + r) : f4v{0.f, 0.f, 0.f, 0.f};
+    }
+    const int64_t j0 = (int64_t)cg * jchunk;
+    const int64_t j1 = (j0 + jchunk < cols) ? j0 + jchunk : cols;
+    for (int64_t j = j0; j < j1; j += UNR) {
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+            *(f4v *)&sb[u][2 * t] = (row >= 0 && j + u < j1) ? *(const f4v *)(B + row + (j + u) * ldb)
+                                                             : f4v{0.f, 0.f, 0.f, 0.f};
+        __syncthreads();
+        float2 a0[UNR], a1[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) a0[u] = a1[u] = czero<float2>();
+#pragma unroll
+        for (int p = 0; p < MAXW; ++p) {
+            if (c[p].x >= 0) {
+                const float2 x = make_float2(v[p].x, v[p].y);
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) a0[u] = cfma(x, sb[u][c[p].x], a0[u]);
+            }
+            if (c[p].y >= 0) {
+                const float2 x = make_float2(v[p].z, v[p].w);
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) a1[u] = cfma(x, sb[u][c[p].y], a1[u]);
+            }
+        }
+        __syncthreads();  // every lane has read this batch before the next one is written
+        if (row >= 0) {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                if (j + u < j1) {
+                    float2 *o = R + row + (j + u) * ldr;
+                    const float2 r0 = axpby(alpha, a0[u], beta, o, (beta_zero & 1) != 0);
+                    const float2 r1 = axpby(alpha, a1[u], beta, o + 1, (beta_zero & 1) != 0);
+                    const f4v q = {r0.x, r0.y, r1.x, r1.y};
+                    if (beta_zero & 2) __builtin_nontemporal_store(q, (f4v *)o);
+                    else *(f4v *)o = q;
+                }
+            }
+        }
+    }
+}
+
 // ---- right, short stripes: a wave covers RR rows x (64 / RR) columns per
 // step (lane l: row l % RR, column l / RR).  Workgroups of one XCD sweep a
 // stripe of RR rows across all columns, so a column of B re-read by the
@@ -503,6 +578,33 @@ static void sell_closed(const SellDev &X, const void *B, int64_t cols, void *R, 
 #undef QOB_CL
 }
 
+static void sell_closed_x2(const SellDev &X, const void *B, int64_t cols, void *R, double2 alpha, double2 beta,
+                           int bz, hipStream_t s, uint32_t nsg, uint32_t G, int jchunk) {
+#define QOB_CL(MW, U)                                                                                          \
+    do {                                                                                                       \
+        if (X.spb == 5)                                                                                        \
+            hipLaunchKernelGGL((k_sell_left_closed_x2<MW, U, 5>), dim3(G), dim3(160), 0, s, X.nslices,         \
+                               X.slice_off, X.rows, X.col, (const float2 *)X.val, (const float2 *)B, X.n, cols, \
+                               (float2 *)R, X.m, alpha, beta, bz, nsg, jchunk, g_sell_cremap);                 \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_sell_left_closed_x2<MW, U, 4>), dim3(G), dim3(128), 0, s, X.nslices,         \
+                               X.slice_off, X.rows, X.col, (const float2 *)X.val, (const float2 *)B, X.n, cols, \
+                               (float2 *)R, X.m, alpha, beta, bz, nsg, jchunk, g_sell_cremap);                 \
+    } while (0)
+#define QOB_CL_U(MW)                         \
+    do {                                     \
+        if (jchunk == 1) QOB_CL(MW, 1);      \
+        else if (jchunk == 2) QOB_CL(MW, 2); \
+        else QOB_CL(MW, 4);                  \
+    } while (0)
+    if (X.maxw <= 2) QOB_CL_U(2);
+    else if (X.maxw <= 4) QOB_CL_U(4);
+    else if (X.maxw <= 8) QOB_CL_U(8);
+    else QOB_CL_U(16);
+#undef QOB_CL_U
+#undef QOB_CL
+}
+
 hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64_t cols, void *R,
                             double2 alpha, double2 beta, hipStream_t s) {
     if (X.m == 0 || cols == 0) return hipSuccess;
@@ -515,6 +617,8 @@ hipError_t launch_sell_left(qob_dtype dt, const SellDev &X, const void *B, int64
         const uint64_t G = (uint64_t)nsg * ncg;
         if (G > 0x7fffffffull) return hipErrorInvalidConfiguration;
         if (dt == QOB_C64) sell_closed<double2>(X, B, cols, R, alpha, beta, bz, s, nsg, (uint32_t)G, jchunk);
+        else if (X.pairs && X.m % 2 == 0 && X.n % 2 == 0 && (((uintptr_t)B | (uintptr_t)R) & 15) == 0)
+            sell_closed_x2(X, B, cols, R, alpha, beta, bz, s, nsg, (uint32_t)G, jchunk);
         else sell_closed<float2>(X, B, cols, R, alpha, beta, bz, s, nsg, (uint32_t)G, jchunk);
         return hipGetLastError();
     }

@@ -213,6 +213,9 @@ struct SellDev {
     // spb.
     const int32_t *rows = nullptr;
     int spb = 4;
+    // Closed blocks whose lanes 2t and 2t + 1 hold rows 2i and 2i + 1 (or
+    // are both idle): ComplexF32 then runs two rows per thread.
+    bool pairs = false;
 };
 // CSC of X (columns of X as lists of (row, value)).
 struct CscDev {

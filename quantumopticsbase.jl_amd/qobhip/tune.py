@@ -28,7 +28,8 @@ of libqob_hip.so (qob_tune_set / qob_tune_get):
                     sell_cjchunk / sell_cremap (closed-block left kernel),
                     nt_store; sell_align (diagonal-aligned slots),
                     sell_closed (closed blocks), sell_cline (closed blocks
-                    of whole 128-byte lines, 320 rows) and csc_groups
+                    of whole 128-byte lines, 320 rows), sell_cpair
+                    (ComplexF32 closed blocks of row pairs) and csc_groups
                     (component groups) apply to tables built after they are
                     set
 

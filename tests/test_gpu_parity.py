@@ -795,6 +795,82 @@ def test_sparse_left_closed_line_aligned(adj):
         assert np.array_equal(outs[0], o)
 
 
+def _chains_matrix(g, n, length, stride):
+    """Square sparse X made of chains of `length` rows: row i couples with
+    row i + stride inside its chain of rows {c, c + stride, ...}; chain c
+    starts at c, so aligned pairs of rows fall into different chains unless
+    they are joined as pairs.  Up to 4 entries per row (diagonal, both
+    neighbours, and a coupling two steps along)."""
+    rows, cols = [], []
+    for i in range(n):
+        rows.append(i); cols.append(i)
+        for d in (stride, -stride, 2 * stride):
+            j = i + d
+            if 0 <= j < n and (j // (stride * length)) == (i // (stride * length)):
+                rows.append(i); cols.append(j)
+    v = g.standard_normal(len(rows)) + 1j * g.standard_normal(len(rows))
+    X = sp.csc_matrix((v, (rows, cols)), shape=(n, n))
+    X.sort_indices()
+    return X
+
+
+@pytest.mark.parametrize("case", ["pairs256", "pairs320", "odd"])
+@pytest.mark.parametrize("adj", [False, True])
+def test_sparse_left_closed_row_pairs(case, adj):
+    """ComplexF32 closed blocks with rows also joined in aligned pairs, so a
+    thread moves rows 2i and 2i + 1 with 16-byte accesses: 256-row blocks
+    (components of 16 rows, the last block partly idle), 320-row blocks
+    (components of 160 rows, which fill 256-row blocks to 62%) and an odd row
+    count (no pairs, one row per lane).  Against the oracle with beta = 0
+    over NaN and beta != 0, and bit for bit against the one-row-per-lane
+    closed form and the contiguous slices."""
+    import ctypes as C
+    from qobhip import _lib
+    g = np.random.default_rng({"pairs256": 31, "pairs320": 32, "odd": 33}[case])
+    if case == "pairs256":
+        X = _line_groups_matrix(g, n=1232, shift=611)
+        big, fill = _line_layout_fill(X, grp=2, blk=256)
+        assert big <= 256 and fill >= 0.9 and X.shape[0] % 256, (big, fill)
+    elif case == "pairs320":
+        X = _chains_matrix(g, 1600, 80, 2)   # pairs join two chains of 80 rows
+        assert _line_layout_fill(X, grp=2, blk=256)[1] < 0.9
+        big, fill = _line_layout_fill(X, grp=2, blk=320)
+        assert big == 160 and fill >= 0.9, (big, fill)
+    else:
+        X = _chains_matrix(g, 1001, 7, 3)
+    n, cols = X.shape[0], 11
+    oop = O.OSparse.from_scipy(X)
+    if adj:
+        oop = O.dagger(oop)
+    B = g.standard_normal((n, cols)) + 1j * g.standard_normal((n, cols))
+    names = (b"sell_closed", b"sell_cpair")
+    saved = {}
+    for name in names:
+        v = C.c_int64()
+        _lib.call("qob_tune_get", name, C.byref(v))
+        saved[name] = v.value
+    outs = []
+    try:
+        for closed, cpair in ((1, 1), (1, 0), (0, 0)):
+            _lib.call("qob_tune_set", b"sell_closed", closed)
+            _lib.call("qob_tune_set", b"sell_cpair", cpair)
+            qop = to_q(oop)   # tables are built on first use, under these settings
+            for alpha, beta in [(1.0, 0.0), (0.3 - 0.8j, 0.5 + 0.25j)]:
+                R0 = np.full((n, cols), np.nan, complex) if beta == 0 else g.standard_normal((n, cols)) + 0j
+                want = np.zeros((n, cols), complex) if beta == 0 else R0.copy()
+                O.mul_op_left(want, oop, B, alpha, beta)
+                R = Q.Operator(gb(n), gb(cols), R0, np.complex64)
+                Q.mul_(R, qop, Q.Operator(gb(n), gb(cols), B, np.complex64), alpha, beta)
+                assert rel(R.to_numpy(), want) < TOL32
+                if beta == 0:
+                    outs.append(R.to_numpy())
+    finally:
+        for name, v in saved.items():
+            _lib.call("qob_tune_set", name, v)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
 @pytest.mark.parametrize("sizes", [[1, 2, 3, 4] * 50, [8, 7, 1, 5] * 20, [2] * 60, [9] + [3] * 40])
 @pytest.mark.parametrize("adj", [False, True])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, TOL64), (np.complex64, TOL32)])

@@ -55,7 +55,7 @@ def test_save_load_roundtrip(tmp_path):
 
 def test_invalid_tunable_values_are_rejected():
     from qobhip import ArgumentError
-    for name, bad in [("mode_kernel", 3), ("fused_gpw", 3), ("qubit_split", 2), ("qubit_tbits", 9), ("qubit_remap", 3), ("qubit_ntfar", 9), ("qubit_order", 3), ("qubit_dbuf", 2), ("qubit_pair", 2)]:
+    for name, bad in [("mode_kernel", 3), ("fused_gpw", 3), ("qubit_split", 2), ("qubit_tbits", 9), ("qubit_remap", 3), ("qubit_ntfar", 9), ("qubit_order", 3), ("qubit_dbuf", 2), ("qubit_pair", 2), ("sell_cpair", 2)]:
         with pytest.raises(ArgumentError):
             T.set_(name, bad)
 
