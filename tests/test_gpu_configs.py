@@ -88,6 +88,34 @@ def test_fock_spin_sparse_left_right(fock_spin):
     assert O.rel_frobenius(left.conj().T, right) < 1e-12
 
 
+def test_fock_spin_sparse_left_right_c32(fock_spin):
+    """The same products in ComplexF32 at full size: the left product runs
+    on closed blocks of row pairs (320-row blocks, two rows per thread), the
+    right product on component groups with two rows per lane.  Against the
+    oracle in ComplexF64 at the ComplexF32 tolerance, and (H rho)^dag =
+    rho H for the Hermitian H and rho."""
+    spec, rho = fock_spin
+    n = spec.H.shape[0]
+    H = Q.SparseOperator(spec.basis, spec.basis, spec.H)
+    rho_h = rho.to_numpy()
+    r32 = Q.Operator(rho.basis_l, rho.basis_r, rho_h, np.complex64)
+    oH = O.OSparse.from_scipy(spec.H)
+    rho_q = r32.to_numpy().astype(complex)   # the rounded input the device sees
+    out = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n, np.complex64, (n, n)))
+    Q.mul_(out, H, r32, 1, 0)
+    want = np.zeros((n, n), complex)
+    O.gemm_sp_left(1.0, oH, rho_q, 0.0, want)
+    left = out.to_numpy().astype(complex)
+    assert O.rel_frobenius(left, want) < 1e-5
+    out2 = Q.Operator(rho.basis_l, rho.basis_r, Q.DeviceArray(n * n, np.complex64, (n, n)))
+    Q.mul_(out2, r32, H, 1, 0)
+    want = np.zeros((n, n), complex)
+    O.gemm_sp_right(1.0, rho_q, oH, 0.0, want)
+    right = out2.to_numpy().astype(complex)
+    assert O.rel_frobenius(right, want) < 1e-5
+    assert O.rel_frobenius(left.conj().T, right) < 1e-5
+
+
 def test_fock_spin_right_column_order_bit_exact(fock_spin):
     """The right product runs on component groups, or sweeps columns in
     reverse Cuthill-McKee order by row stripes or column-major; each output
