@@ -852,7 +852,7 @@ def main():
                     help="fixed short launch sequence for rocprofv3 counter runs (copy, then the workload's kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU "
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library has no multi-GPU "
               f"path: run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     ds, dw = DEFAULT_STEPS[args.workload]
