@@ -188,30 +188,74 @@ __global__ void __launch_bounds__(REG_NT) k_lt_reg(const cplx<R> *__restrict__ A
             if (PIPE && ncol < ncols && wave < nblk)
                 block_load<R, D2>(in + ncol * (int64_t)D + (int64_t)wave * BLK, lane, xa);
             // mode 3: Z^T (k3' x e) = A3 Y^T over groups of 16 elements e
-            for (int g = wave; g < BLK / 16; g += WPB) {
-                const int e0 = 16 * g;
-                V zr[NT3], zi[NT3];
+            if constexpr (PIPE && NT3 == 1) {
+                // ComplexF64 with one output tile per group: every wave runs
+                // the same compile-time number of groups, so the wait for the
+                // next column's loads (issued above, before this mode's
+                // stores) counts all of the stores that follow them.  With a
+                // run-time trip count the compiler assumes one group's stores
+                // and that wait also waited for 12 of the 16 (1% of the
+                // kernel's time).  The scheduling fence keeps one group's
+                // registers live at a time; with two tiles per group (a
+                // 32-wide site 3) the unrolled form spills, so that shape
+                // keeps the loop.
+                auto group3 = [&](int g) {
+                    const int e0 = 16 * g;
+                    V zr[NT3], zi[NT3];
 #pragma unroll
-                for (int v = 0; v < NT3; ++v) {
-                    zr[v] = V{0, 0, 0, 0};
-                    zi[v] = V{0, 0, 0, 0};
-                }
-#pragma unroll
-                for (int ks = 0; ks < KS3; ++ks) {
-                    const int idx = pad16r(e0 + lr + BLK * (4 * ks + lk));
-                    const T yv = mk(yre[idx], yim[idx]);
-#pragma unroll
-                    for (int v = 0; v < NT3; ++v) cmma4<R>(a3f[v][ks], yv, zr[v], zi[v]);
-                }
-#pragma unroll
-                for (int v = 0; v < NT3; ++v)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        T *o = oc + e0 + lr + (int64_t)BLK * (16 * v + mf<R>::row(lane, r));
-                        T val = cmul(alpha, mk((R)zr[v][r], (R)zi[v][r]));
-                        if (!beta_zero) val = cfma(beta, *o, val);
-                        st_stream(o, val);
+                    for (int v = 0; v < NT3; ++v) {
+                        zr[v] = V{0, 0, 0, 0};
+                        zi[v] = V{0, 0, 0, 0};
                     }
+#pragma unroll
+                    for (int ks = 0; ks < KS3; ++ks) {
+                        const int idx = pad16r(e0 + lr + BLK * (4 * ks + lk));
+                        const T yv = mk(yre[idx], yim[idx]);
+#pragma unroll
+                        for (int v = 0; v < NT3; ++v) cmma4<R>(a3f[v][ks], yv, zr[v], zi[v]);
+                    }
+#pragma unroll
+                    for (int v = 0; v < NT3; ++v)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            T *o = oc + e0 + lr + (int64_t)BLK * (16 * v + mf<R>::row(lane, r));
+                            T val = cmul(alpha, mk((R)zr[v][r], (R)zi[v][r]));
+                            if (!beta_zero) val = cfma(beta, *o, val);
+                            st_stream(o, val);
+                        }
+                };
+                static_assert((BLK / 16) % WPB == 0, "mode-3 groups per wave");
+#pragma unroll
+                for (int q = 0; q < BLK / 16 / WPB; ++q) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    group3(wave + q * WPB);
+                }
+            } else {
+                for (int g = wave; g < BLK / 16; g += WPB) {
+                    const int e0 = 16 * g;
+                    V zr[NT3], zi[NT3];
+#pragma unroll
+                    for (int v = 0; v < NT3; ++v) {
+                        zr[v] = V{0, 0, 0, 0};
+                        zi[v] = V{0, 0, 0, 0};
+                    }
+#pragma unroll
+                    for (int ks = 0; ks < KS3; ++ks) {
+                        const int idx = pad16r(e0 + lr + BLK * (4 * ks + lk));
+                        const T yv = mk(yre[idx], yim[idx]);
+#pragma unroll
+                        for (int v = 0; v < NT3; ++v) cmma4<R>(a3f[v][ks], yv, zr[v], zi[v]);
+                    }
+#pragma unroll
+                    for (int v = 0; v < NT3; ++v)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            T *o = oc + e0 + lr + (int64_t)BLK * (16 * v + mf<R>::row(lane, r));
+                            T val = cmul(alpha, mk((R)zr[v][r], (R)zi[v][r]));
+                            if (!beta_zero) val = cfma(beta, *o, val);
+                            st_stream(o, val);
+                        }
+                }
             }
         }
     }
