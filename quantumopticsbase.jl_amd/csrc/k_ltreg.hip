@@ -363,8 +363,16 @@ __global__ void __launch_bounds__(REG_NT) k_lt_reg_tile(const cplx<R> *__restric
         }
         __syncthreads();  // the tile's planes are complete
         if (nc < ntiles) load_tile(tile_base(nc));
-        // mode 3 over groups of 16 plane elements e = r + 16 i2 (r = lane % 16)
-        for (int g = wave; g < BLK / 16; g += WPB) {
+        // mode 3 over groups of 16 plane elements e = r + 16 i2 (r = lane % 16).
+        // A compile-time group count per wave: the wait for the next tile's
+        // loads (issued above, before these stores) then counts every store
+        // of this mode, where a run-time trip count made the next tile's
+        // first MFMAs wait for all but one group's stores as well.
+        static_assert((BLK / 16) % WPB == 0, "mode-3 groups per wave");
+#pragma unroll
+        for (int q = 0; q < BLK / 16 / WPB; ++q) {
+            __builtin_amdgcn_sched_barrier(0);  // one group's registers live at a time
+            const int g = wave + q * WPB;
             const int e0 = 16 * g;
             V zr[NT3], zi[NT3];
 #pragma unroll
