@@ -103,11 +103,19 @@ __global__ void __launch_bounds__(MODE_NT) k_mode_fib(const cplx<R> *__restrict_
         if (!ok) return {nullptr, 0};
         return {in + i1 + i3 * sin, i1 + i3 * sout};
     };
+    // Every lane issues all of its loads (from an in-bounds address where
+    // the element does not exist, followed by a zero select): with loads
+    // under a branch the compiler must assume that none was issued after
+    // the chunk in use, and waited for the next chunk's loads before this
+    // chunk's MFMAs.  (16, 32, 16) chain on an 8192^2 rho: 1.211 -> 1.185 ms.
     auto load = [&](const FibreAt &fa, int kc, T (&x)[4]) {
+        const T *base = fa.p ? (const T *)fa.p : in;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int b = 16 * kc + 4 * j + lk;
-            x[j] = (fa.p && b < d_in) ? ((const T *)fa.p)[(int64_t)n1 * b] : czero<T>();
+            const bool ok = fa.p && b < d_in;
+            const T v = ldg(base + (int64_t)n1 * (ok ? b : 0));
+            x[j] = ok ? v : czero<T>();
         }
     };
     int64_t g = (int64_t)blockIdx.x * WPB + wave;
