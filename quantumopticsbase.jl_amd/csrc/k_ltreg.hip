@@ -231,6 +231,8 @@ __global__ void __launch_bounds__(REG_NT) k_lt_reg(const cplx<R> *__restrict__ A
                     group3(wave + q * WPB);
                 }
             } else {
+                // the body of group3, written out: calling the lambda here
+                // costs the ComplexF32 forms 4-9 more VGPRs
                 for (int g = wave; g < BLK / 16; g += WPB) {
                     const int e0 = 16 * g;
                     V zr[NT3], zi[NT3];
